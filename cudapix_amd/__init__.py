@@ -1,0 +1,558 @@
+"""cudapix_amd -- MI355X combinatorial pi(x) (Deleglise-Rivat) engine.
+
+Python mirror of the reference's pi(x) interface over the C ABI of
+libcudapix_amd.so (include/cudapix.h).  Function names follow the reference
+headers they replace: P2 (include/P2.cuh:16-17), Ordinary (include/ordinary.cuh:6-7),
+S2_trivial (include/trivial.cuh:8-12), S3 (include/S3.cuh:3), V / V_a / V_b
+(include/V.cuh:6-8), get_d_piTable / PiTable (include/pitable.cuh:7-51),
+Phi (include/phi.cuh:17-36), gen_d_lpf / gen_d_mu (include/sieve/lpf_mu.cuh:14-21),
+countPrimes / getDevicePrimes (CUDASieve), and pi_deleglise_rivat
+(src/Deleglise-Rivat/deleglise-rivat.cpp:50).
+
+Every computation runs in the HIP library on the GPU; there is no CPU
+fallback.  Importing works without a GPU (the library loads and its host-side
+self checks run); creating a Context needs a HIP device.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcudapix_amd.so")
+
+U64 = ctypes.c_uint64
+MASK64 = (1 << 64) - 1
+
+
+class CpxU128(ctypes.Structure):
+    _fields_ = [("lo", U64), ("hi", U64)]
+
+
+CpxI128 = CpxU128
+
+
+class CpxOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("c", ctypes.c_uint32), ("alpha", ctypes.c_double),
+                ("seg_threads", ctypes.c_uint32), ("seg_kwords", ctypes.c_uint32),
+                ("hbm_budget", U64), ("fresh_tables", ctypes.c_uint32),
+                ("use_graphs", ctypes.c_uint32)]
+
+
+class CpxTerms(ctypes.Structure):
+    _fields_ = [("pi", CpxU128), ("ordinary", CpxI128), ("trivial", CpxI128),
+                ("easy", CpxI128), ("hard", CpxI128), ("p2", CpxI128),
+                ("y", U64), ("z", U64), ("c", U64), ("pi_y", U64),
+                ("alpha", ctypes.c_double), ("ms", ctypes.c_double)]
+
+
+class CpxBatchOpts(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_uint32), ("tune", ctypes.c_uint32)]
+
+
+class CpxBatchStats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_plan", ctypes.c_double),
+                ("ms_tables", ctypes.c_double), ("ms_device", ctypes.c_double),
+                ("ms_segments", ctypes.c_double), ("tiles", U64), ("hard_segments", U64),
+                ("p2_segments", U64), ("seg_threads", ctypes.c_uint32),
+                ("seg_kwords", ctypes.c_uint32)]
+
+
+class CpxServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", U64), ("max_batch", ctypes.c_uint32),
+                ("max_wait_ms", ctypes.c_double)]
+
+
+class CpxResult(ctypes.Structure):
+    _fields_ = [("ticket", U64), ("x", CpxU128), ("pi", CpxU128), ("status", ctypes.c_int),
+                ("t_arrive_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double),
+                ("t_done_ms", ctypes.c_double)]
+
+
+class CpxServerStats(ctypes.Structure):
+    _fields_ = [("submitted", U64), ("completed", U64), ("rejected", U64), ("cancelled", U64),
+                ("batches", U64), ("max_inflight_bytes", U64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("qps", ctypes.c_double)]
+
+
+# every symbol include/cudapix.h declares (checked by the CPU tests)
+EXPORTS = [
+    "cpx_opts_default", "cpx_ctx_create", "cpx_ctx_destroy", "cpx_strerror",
+    "cpx_ctx_last_error", "cpx_choose_y", "cpx_ctx_device_bytes",
+    "cpx_pi_u64", "cpx_pi_u128",
+    "cpx_p2_u64", "cpx_p2_u128", "cpx_div_x_by_y",
+    "cpx_ordinary_u64", "cpx_ordinary_u128",
+    "cpx_s2_trivial_u64", "cpx_s2_trivial_u128",
+    "cpx_s3_u64", "cpx_s3_u128", "cpx_v_u64", "cpx_v_u128", "cpx_v_split_u64",
+    "cpx_count_primes", "cpx_get_device_primes", "cpx_get_d_pitable",
+    "cpx_pitable_create", "cpx_pitable_destroy", "cpx_pitable_current",
+    "cpx_pitable_next_up", "cpx_pitable_next_down", "cpx_pitable_info",
+    "cpx_phi_create", "cpx_phi_destroy", "cpx_phi_eval", "cpx_phi_range",
+    "cpx_gen_d_lpf", "cpx_gen_d_mu", "cpx_free",
+    "cpx_pi_batch", "cpx_pi_batch_terms", "cpx_autotune",
+    "cpx_server_create", "cpx_server_submit", "cpx_server_cancel", "cpx_server_poll",
+    "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
+    "cpx_request_bytes", "cpx_selftest_arith",
+]
+
+ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
+          -5: "CPX_EAGAIN", -6: "CPX_ECANCELED", -7: "CPX_ENOTFOUND", -8: "CPX_ESTATE"}
+
+_lib = None
+
+
+class CpxError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+
+
+def lib():
+    """Load libcudapix_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libcudapix_amd.so is not built (run __graft_entry__.build()); "
+                          "there is no fallback path")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    sig = {
+        "cpx_opts_default": (None, [P(CpxOpts)]),
+        "cpx_ctx_create": (ctypes.c_int, [P(CpxOpts), P(vp)]),
+        "cpx_ctx_destroy": (ctypes.c_int, [vp]),
+        "cpx_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+        "cpx_ctx_last_error": (ctypes.c_char_p, [vp]),
+        "cpx_choose_y": (ctypes.c_int, [vp, CpxU128, P(U64)]),
+        "cpx_ctx_device_bytes": (U64, [vp]),
+        "cpx_pi_u64": (ctypes.c_int, [vp, U64, U64, ctypes.c_uint32, P(U64), P(CpxTerms)]),
+        "cpx_pi_u128": (ctypes.c_int, [vp, CpxU128, U64, ctypes.c_uint32, P(CpxU128), P(CpxTerms)]),
+        "cpx_p2_u64": (ctypes.c_int, [vp, U64, U64, P(U64)]),
+        "cpx_p2_u128": (ctypes.c_int, [vp, CpxU128, U64, P(CpxU128)]),
+        "cpx_div_x_by_y": (ctypes.c_int, [vp, CpxU128, vp, ctypes.c_size_t]),
+        "cpx_ordinary_u64": (ctypes.c_int, [vp, U64, U64, ctypes.c_uint16, P(ctypes.c_int64)]),
+        "cpx_ordinary_u128": (ctypes.c_int, [vp, CpxU128, U64, ctypes.c_uint16, P(CpxI128)]),
+        "cpx_s2_trivial_u64": (ctypes.c_int, [vp, U64, U64, P(U64)]),
+        "cpx_s2_trivial_u128": (ctypes.c_int, [vp, CpxU128, U64, P(CpxU128)]),
+        "cpx_s3_u64": (ctypes.c_int, [vp, U64, U64, ctypes.c_uint32, P(ctypes.c_int64)]),
+        "cpx_s3_u128": (ctypes.c_int, [vp, CpxU128, U64, ctypes.c_uint32, P(CpxI128)]),
+        "cpx_v_u64": (ctypes.c_int, [vp, U64, U64, P(ctypes.c_int64)]),
+        "cpx_v_u128": (ctypes.c_int, [vp, CpxU128, U64, P(CpxI128)]),
+        "cpx_v_split_u64": (ctypes.c_int, [vp, U64, U64, P(ctypes.c_int64), P(ctypes.c_int64)]),
+        "cpx_count_primes": (ctypes.c_int, [vp, U64, U64, P(U64)]),
+        "cpx_get_device_primes": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
+        "cpx_get_d_pitable": (ctypes.c_int, [vp, ctypes.c_uint32, P(vp)]),
+        "cpx_pitable_create": (ctypes.c_int, [vp, U64, U64, U64, P(vp)]),
+        "cpx_pitable_destroy": (ctypes.c_int, [vp]),
+        "cpx_pitable_current": (ctypes.c_int, [vp, P(vp)]),
+        "cpx_pitable_next_up": (ctypes.c_int, [vp, P(vp)]),
+        "cpx_pitable_next_down": (ctypes.c_int, [vp, P(vp)]),
+        "cpx_pitable_info": (ctypes.c_int, [vp, P(U64), P(U64), P(U64)]),
+        "cpx_phi_create": (ctypes.c_int, [vp, ctypes.c_uint32, P(vp)]),
+        "cpx_phi_destroy": (ctypes.c_int, [vp]),
+        "cpx_phi_eval": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, P(U64)]),
+        "cpx_phi_range": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, P(vp)]),
+        "cpx_gen_d_lpf": (ctypes.c_int, [vp, U64, U64, P(vp)]),
+        "cpx_gen_d_mu": (ctypes.c_int, [vp, U64, U64, P(vp)]),
+        "cpx_free": (ctypes.c_int, [vp, vp]),
+        "cpx_pi_batch": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128),
+                                        P(CpxBatchOpts), P(CpxBatchStats)]),
+        "cpx_pi_batch_terms": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128),
+                                              P(CpxTerms), P(CpxBatchOpts), P(CpxBatchStats)]),
+        "cpx_autotune": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(ctypes.c_uint32),
+                                        P(ctypes.c_uint32)]),
+        "cpx_server_create": (ctypes.c_int, [vp, P(CpxServerOpts), P(vp)]),
+        "cpx_server_submit": (ctypes.c_int, [vp, CpxU128, P(U64)]),
+        "cpx_server_cancel": (ctypes.c_int, [vp, U64]),
+        "cpx_server_poll": (ctypes.c_int, [vp, P(CpxResult), ctypes.c_size_t, P(ctypes.c_size_t)]),
+        "cpx_server_drain": (ctypes.c_int, [vp]),
+        "cpx_server_stats_get": (ctypes.c_int, [vp, P(CpxServerStats)]),
+        "cpx_server_destroy": (ctypes.c_int, [vp]),
+        "cpx_server_replay": (ctypes.c_int, [vp, P(CpxU128), P(ctypes.c_double), ctypes.c_size_t,
+                                             P(U64)]),
+        "cpx_request_bytes": (U64, [vp, CpxU128]),
+        "cpx_selftest_arith": (ctypes.c_int, [U64, U64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def u128(v):
+    if v < 0 or v >> 128:
+        raise ValueError("x must be a non-negative 128-bit integer")
+    return CpxU128(v & MASK64, v >> 64)
+
+
+def from_u128(s):
+    return (s.hi << 64) | s.lo
+
+
+def from_i128(s):
+    v = (s.hi << 64) | s.lo
+    return v - (1 << 128) if v >> 127 else v
+
+
+def _terms_dict(t):
+    return {"pi": from_u128(t.pi), "ordinary": from_i128(t.ordinary),
+            "trivial": from_i128(t.trivial), "easy": from_i128(t.easy),
+            "hard": from_i128(t.hard), "p2": from_i128(t.p2), "y": t.y, "z": t.z,
+            "c": t.c, "pi_y": t.pi_y, "alpha": t.alpha, "ms": t.ms}
+
+
+class Context:
+    """A device context: resident tables + stream (one per thread of use)."""
+
+    def __init__(self, device=0, c=6, alpha=0.0, seg_threads=0, seg_kwords=0, hbm_budget=0,
+                 fresh_tables=False, use_graphs=True):
+        L = lib()
+        o = CpxOpts()
+        L.cpx_opts_default(ctypes.byref(o))
+        o.device, o.c, o.alpha = device, c, alpha
+        o.seg_threads, o.seg_kwords, o.hbm_budget = seg_threads, seg_kwords, hbm_budget
+        o.fresh_tables, o.use_graphs = int(fresh_tables), int(use_graphs)
+        h = ctypes.c_void_p()
+        rc = L.cpx_ctx_create(ctypes.byref(o), ctypes.byref(h))
+        if rc:
+            raise CpxError(rc, "cpx_ctx_create")
+        self.h = h
+        self._L = L
+
+    def close(self):
+        if self.h:
+            self._L.cpx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self._L.cpx_ctx_last_error(self.h)
+            raise CpxError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    # ---- full pi(x)
+    def pi(self, x, y=0, c=0, terms=False):
+        o = CpxU128()
+        t = CpxTerms()
+        self._chk(self._L.cpx_pi_u128(self.h, u128(x), y, c, ctypes.byref(o), ctypes.byref(t)),
+                  "pi")
+        return _terms_dict(t) if terms else from_u128(o)
+
+    def choose_y(self, x):
+        y = U64()
+        self._chk(self._L.cpx_choose_y(self.h, u128(x), ctypes.byref(y)), "choose_y")
+        return y.value
+
+    # ---- terms
+    def P2(self, x, y):
+        o = CpxU128()
+        self._chk(self._L.cpx_p2_u128(self.h, u128(x), y, ctypes.byref(o)), "P2")
+        return from_u128(o)
+
+    def Ordinary(self, x, y, c=6):
+        o = CpxI128()
+        self._chk(self._L.cpx_ordinary_u128(self.h, u128(x), y, c, ctypes.byref(o)), "Ordinary")
+        return from_i128(o)
+
+    def S2_trivial(self, x, y):
+        o = CpxU128()
+        self._chk(self._L.cpx_s2_trivial_u128(self.h, u128(x), y, ctypes.byref(o)), "S2_trivial")
+        return from_u128(o)
+
+    def S3(self, x, y, c=6):
+        o = CpxI128()
+        self._chk(self._L.cpx_s3_u128(self.h, u128(x), y, c, ctypes.byref(o)), "S3")
+        return from_i128(o)
+
+    def V(self, x, y):
+        o = CpxI128()
+        self._chk(self._L.cpx_v_u128(self.h, u128(x), y, ctypes.byref(o)), "V")
+        return from_i128(o)
+
+    def V_split(self, x, y):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._L.cpx_v_split_u64(self.h, x, y, ctypes.byref(a), ctypes.byref(b)),
+                  "V_split")
+        return a.value, b.value
+
+    # ---- primes / tables
+    def countPrimes(self, lo, hi):
+        o = U64()
+        self._chk(self._L.cpx_count_primes(self.h, lo, hi, ctypes.byref(o)), "countPrimes")
+        return o.value
+
+    def _d2h(self, dptr, n, ctype):
+        """Copy a device array into a Python list (through hipMemcpy in libamdhip64)."""
+        buf = (ctype * max(n, 1))()
+        hip = _hip()
+        rc = hip.hipMemcpy(buf, dptr, ctypes.sizeof(ctype) * n, 2)  # DeviceToHost
+        if rc:
+            raise CpxError(-3, "hipMemcpy %d" % rc)
+        return list(buf[:n])
+
+    def getDevicePrimes(self, lo, hi):
+        d = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        self._chk(self._L.cpx_get_device_primes(self.h, lo, hi, ctypes.byref(d), ctypes.byref(n)),
+                  "getDevicePrimes")
+        try:
+            return self._d2h(d, n.value, U64)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def get_d_piTable(self, hi):
+        d = ctypes.c_void_p()
+        self._chk(self._L.cpx_get_d_pitable(self.h, hi, ctypes.byref(d)), "get_d_piTable")
+        try:
+            return self._d2h(d, hi // 2, ctypes.c_uint32)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def gen_lpf(self, bottom, top):
+        d = ctypes.c_void_p()
+        self._chk(self._L.cpx_gen_d_lpf(self.h, bottom, top, ctypes.byref(d)), "gen_d_lpf")
+        try:
+            return self._d2h(d, (top - bottom) // 2, ctypes.c_uint32)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def gen_mu(self, bottom, top):
+        d = ctypes.c_void_p()
+        self._chk(self._L.cpx_gen_d_mu(self.h, bottom, top, ctypes.byref(d)), "gen_d_mu")
+        try:
+            return self._d2h(d, (top - bottom) // 2, ctypes.c_int8)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def divXbyY(self, x, ys):
+        hip = _hip()
+        n = len(ys)
+        arr = (U64 * max(n, 1))(*ys)
+        d = ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(d), 8 * max(n, 1)):
+            raise CpxError(-2, "hipMalloc")
+        try:
+            hip.hipMemcpy(d, arr, 8 * n, 1)
+            self._chk(self._L.cpx_div_x_by_y(self.h, u128(x), d, n), "divXbyY")
+            return self._d2h(d, n, U64)
+        finally:
+            hip.hipFree(d)
+
+    # ---- batch
+    def pi_batch(self, xs, use_graph=True, tune=False, terms=False, stats=False):
+        n = len(xs)
+        arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
+        out = (CpxU128 * max(n, 1))()
+        bo = CpxBatchOpts(int(use_graph), int(tune))
+        st = CpxBatchStats()
+        if terms:
+            tt = (CpxTerms * max(n, 1))()
+            rc = self._L.cpx_pi_batch_terms(self.h, arr, n, out, tt, ctypes.byref(bo),
+                                            ctypes.byref(st))
+        else:
+            rc = self._L.cpx_pi_batch(self.h, arr, n, out, ctypes.byref(bo), ctypes.byref(st))
+        self._chk(rc, "pi_batch")
+        res = [_terms_dict(tt[i]) for i in range(n)] if terms else \
+            [from_u128(out[i]) for i in range(n)]
+        if stats:
+            return res, {f: getattr(st, f) for f, _ in CpxBatchStats._fields_}
+        return res
+
+    def autotune(self, xs):
+        n = len(xs)
+        arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
+        t, k = ctypes.c_uint32(), ctypes.c_uint32()
+        self._chk(self._L.cpx_autotune(self.h, arr, n, ctypes.byref(t), ctypes.byref(k)),
+                  "autotune")
+        return t.value, k.value
+
+    def request_bytes(self, x):
+        return self._L.cpx_request_bytes(self.h, u128(x))
+
+    def device_bytes(self):
+        return self._L.cpx_ctx_device_bytes(self.h)
+
+
+class PiTable:
+    """Sliding pi-table window (class PiTable, include/pitable.cuh:12-51)."""
+
+    def __init__(self, ctx, base, bottom, range_):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._chk(ctx._L.cpx_pitable_create(ctx.h, base, bottom, range_, ctypes.byref(h)),
+                 "PiTable")
+        self.h = h
+
+    def _get(self, fn):
+        d = ctypes.c_void_p()
+        self.ctx._chk(fn(self.h, ctypes.byref(d)), "PiTable window")
+        base, rng, pb = U64(), U64(), U64()
+        self.ctx._L.cpx_pitable_info(self.h, ctypes.byref(base), ctypes.byref(rng),
+                                     ctypes.byref(pb))
+        vals = self.ctx._d2h(d, rng.value // 2 + 1, ctypes.c_uint32)
+        return base.value, pb.value, vals
+
+    def getCurrent(self):
+        return self._get(self.ctx._L.cpx_pitable_current)
+
+    def getNextUp(self):
+        return self._get(self.ctx._L.cpx_pitable_next_up)
+
+    def getNextDown(self):
+        return self._get(self.ctx._L.cpx_pitable_next_down)
+
+    def close(self):
+        if self.h:
+            self.ctx._L.cpx_pitable_destroy(self.h)
+            self.h = None
+
+
+class Phi:
+    """phi(x, a) for x < max from a device LPF table (class Phi, include/phi.cuh:17-36)."""
+
+    def __init__(self, ctx, max_):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._chk(ctx._L.cpx_phi_create(ctx.h, max_, ctypes.byref(h)), "Phi")
+        self.h = h
+
+    def phi(self, x, a):
+        o = U64()
+        self.ctx._chk(self.ctx._L.cpx_phi_eval(self.h, x, a, ctypes.byref(o)), "Phi.phi")
+        return o.value
+
+    def generateRange(self, num, a):
+        d = ctypes.c_void_p()
+        self.ctx._chk(self.ctx._L.cpx_phi_range(self.h, num, a, ctypes.byref(d)),
+                      "Phi.generateRange")
+        try:
+            return self.ctx._d2h(d, num // 2, ctypes.c_uint32)
+        finally:
+            self.ctx._L.cpx_free(self.ctx.h, d)
+
+    def close(self):
+        if self.h:
+            self.ctx._L.cpx_phi_destroy(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching pi(x) server (admission by predicted HBM bytes)."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch=256, max_wait_ms=0.5):
+        self.ctx = ctx
+        o = CpxServerOpts(hbm_budget, max_batch, max_wait_ms)
+        h = ctypes.c_void_p()
+        ctx._chk(ctx._L.cpx_server_create(ctx.h, ctypes.byref(o), ctypes.byref(h)), "Server")
+        self.h = h
+        self._L = ctx._L
+
+    def submit(self, x):
+        t = U64()
+        rc = self._L.cpx_server_submit(self.h, u128(x), ctypes.byref(t))
+        if rc:
+            raise CpxError(rc, "submit")
+        return t.value
+
+    def cancel(self, ticket):
+        return self._L.cpx_server_cancel(self.h, ticket)
+
+    def replay(self, xs, t_ms):
+        n = len(xs)
+        arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
+        ts = (ctypes.c_double * max(n, 1))(*t_ms)
+        tk = (U64 * max(n, 1))()
+        rc = self._L.cpx_server_replay(self.h, arr, ts, n, tk)
+        if rc:
+            raise CpxError(rc, "replay")
+        return list(tk[:n])
+
+    def drain(self):
+        self._L.cpx_server_drain(self.h)
+
+    def poll(self, max_=4096):
+        out = (CpxResult * max_)()
+        n = ctypes.c_size_t()
+        self._L.cpx_server_poll(self.h, out, max_, ctypes.byref(n))
+        return [{"ticket": r.ticket, "x": from_u128(r.x), "pi": from_u128(r.pi),
+                 "status": r.status, "t_arrive_ms": r.t_arrive_ms, "t_admit_ms": r.t_admit_ms,
+                 "t_done_ms": r.t_done_ms} for r in out[:n.value]]
+
+    def stats(self):
+        s = CpxServerStats()
+        self._L.cpx_server_stats_get(self.h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in CpxServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self._L.cpx_server_destroy(self.h)
+            self.h = None
+
+
+_hip_lib = None
+
+
+def _hip():
+    global _hip_lib
+    if _hip_lib is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipMemcpy.restype = ctypes.c_int
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipMalloc.restype = ctypes.c_int
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipFree.restype = ctypes.c_int
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        _hip_lib = h
+    return _hip_lib
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context()
+    return _default
+
+
+# reference-named module-level functions on a default context
+def P2(x, y):
+    return default_context().P2(x, y)
+
+
+def Ordinary(x, y, c=6):
+    return default_context().Ordinary(x, y, c)
+
+
+def S2_trivial(x, y):
+    return default_context().S2_trivial(x, y)
+
+
+def S3(x, y, c=6):
+    return default_context().S3(x, y, c)
+
+
+def V(x, y):
+    return default_context().V(x, y)
+
+
+def pi_deleglise_rivat(x, y=0, c=6):
+    return default_context().pi(x, y, c)
+
+
+def selftest_arith(seed=1, n=100000):
+    return lib().cpx_selftest_arith(seed, n)

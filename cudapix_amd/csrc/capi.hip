@@ -1,0 +1,729 @@
+// capi.hip -- extern "C" entry points of libcudapix_amd.so (declared in
+// include/cudapix.h; the reference interface each one replaces is cited there).
+#include "engine.hpp"
+#include <chrono>
+#include <cstring>
+#include <random>
+
+void forget_arena(const cpx_ctx *c);
+
+static double now_ms()
+{
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+static inline u128 to_u128(cpx_u128 v) { return ((u128)v.hi << 64) | v.lo; }
+static inline cpx_u128 from_u128(u128 v) { return {(uint64_t)v, (uint64_t)(v >> 64)}; }
+static inline i128 get_i128(const uint64_t a[2]) { return (i128)(((u128)a[1] << 64) | a[0]); }
+static inline cpx_i128 to_ci(const uint64_t a[2]) { return {a[0], a[1]}; }
+
+#define CTX_GUARD(c)                         \
+  if (!(c)) return CPX_EINVAL;               \
+  std::lock_guard<std::recursive_mutex> lk_((c)->mu)
+
+#define HIPCHK(call)                                     \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail(c, e_, #call); \
+  } while (0)
+
+extern "C" {
+
+void cpx_opts_default(cpx_opts *o)
+{
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->device = 0;
+  o->c = 6;
+  o->alpha = 0;
+  o->use_graphs = 1;
+}
+
+const char *cpx_strerror(int code)
+{
+  switch (code) {
+    case CPX_OK: return "ok";
+    case CPX_EINVAL: return "invalid argument";
+    case CPX_ENOMEM: return "out of device memory or over the HBM budget";
+    case CPX_EDEVICE: return "device (HIP) error";
+    case CPX_ERANGE: return "argument out of the supported range";
+    case CPX_EAGAIN: return "no result ready";
+    case CPX_ECANCELED: return "request cancelled";
+    case CPX_ENOTFOUND: return "unknown ticket";
+    case CPX_ESTATE: return "invalid state";
+    default: return "unknown error";
+  }
+}
+
+const char *cpx_ctx_last_error(const cpx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int cpx_ctx_create(const cpx_opts *o, cpx_ctx **out)
+{
+  if (!out) return CPX_EINVAL;
+  *out = nullptr;
+  cpx_opts op;
+  if (o) op = *o; else cpx_opts_default(&op);
+  if (op.c == 0) op.c = 6;
+  if (op.c > 8) return CPX_EINVAL;
+  cpx_ctx *c = new cpx_ctx();
+  c->opts = op;
+  c->device = op.device;
+  hipError_t e = hipSetDevice(op.device);
+  if (e != hipSuccess) { int r = hip_fail(c, e, "hipSetDevice"); delete c; return r; }
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    int r = hip_fail(c, e, "hipStreamCreate"); delete c; return r;
+  }
+  for (auto &ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) { int r = hip_fail(c, e, "hipEventCreate"); cpx_ctx_destroy(c); return r; }
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, op.device)) != hipSuccess) {
+    int r = hip_fail(c, e, "hipGetDeviceProperties"); cpx_ctx_destroy(c); return r;
+  }
+  c->cu_count = prop.multiProcessorCount;
+  if (op.hbm_budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->opts.hbm_budget = (uint64_t)(fr * 0.9);
+  }
+  if (op.seg_threads && op.seg_kwords) {
+    SegShape s{(int)op.seg_threads, (int)op.seg_kwords};
+    if (!seg_shape_supported(s)) { cpx_ctx_destroy(c); return CPX_EINVAL; }
+    c->shape = s;
+    c->shape_tuned = true;
+  }
+  *out = c;
+  return CPX_OK;
+}
+
+int cpx_ctx_destroy(cpx_ctx *c)
+{
+  if (!c) return CPX_EINVAL;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  free_tables(c);
+  if (c->small.p) (void)hipFree(c->small.p);
+  if (c->small_primes.p) (void)hipFree(c->small_primes.p);
+  if (c->wheel.p) (void)hipFree(c->wheel.p);
+  if (c->winfo.p) (void)hipFree(c->winfo.p);
+  if (c->d_plan) (void)hipFree(c->d_plan);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->h_plan) (void)hipHostFree(c->h_plan);
+  if (c->h_res) (void)hipHostFree(c->h_res);
+  for (auto &ev : c->ev) if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  forget_arena(c);
+  delete c;
+  return CPX_OK;
+}
+
+int cpx_choose_y(cpx_ctx *c, cpx_u128 x, uint64_t *y)
+{
+  CTX_GUARD(c);
+  if (!y) return CPX_EINVAL;
+  *y = choose_y_host(c, to_u128(x));
+  return CPX_OK;
+}
+
+uint64_t cpx_ctx_device_bytes(const cpx_ctx *c)
+{
+  if (!c) return 0;
+  return c->table_bytes + c->d_plan_cap + c->d_out_cap;
+}
+
+// -------------------------------------------------------------- one query
+
+static int solve_one(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, uint32_t mask, ResultDev *r,
+                     HostQuery *hq_out = nullptr, double *ms = nullptr)
+{
+  double t0 = now_ms();
+  HIPCHK(hipSetDevice(c->device));
+  HostQuery hq;
+  int rc = make_host_query(c, x, y, cc, &hq);
+  if (rc) return rc;
+  std::vector<HostQuery> v{hq};
+  std::vector<ResultDev> res;
+  if ((rc = run_queries(c, v, mask, res, nullptr, false))) return rc;
+  *r = res[0];
+  if (hq_out) *hq_out = hq;
+  if (ms) *ms = now_ms() - t0;
+  return CPX_OK;
+}
+
+static void fill_terms(cpx_terms *t, const ResultDev &r, const HostQuery &h, double ms)
+{
+  if (!t) return;
+  memset(t, 0, sizeof(*t));
+  t->pi = {r.pi[0], r.pi[1]};
+  t->ordinary = to_ci(r.ordinary);
+  t->trivial = to_ci(r.trivial);
+  t->easy = to_ci(r.easy);
+  t->hard = to_ci(r.hard);
+  t->p2 = to_ci(r.p2);
+  t->y = h.y; t->z = h.z; t->c = h.c; t->pi_y = r.pi_y;
+  t->alpha = h.cbrtx ? (double)h.y / (double)h.cbrtx : 0;
+  t->ms = ms;
+}
+
+int cpx_pi_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, uint32_t cc, cpx_u128 *out, cpx_terms *terms)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  ResultDev r; HostQuery h; double ms = 0;
+  int rc = solve_one(c, to_u128(x), y, cc, TERM_ALL, &r, &h, &ms);
+  if (rc) return rc;
+  *out = {r.pi[0], r.pi[1]};
+  fill_terms(terms, r, h, ms);
+  return CPX_OK;
+}
+
+int cpx_pi_u64(cpx_ctx *c, uint64_t x, uint64_t y, uint32_t cc, uint64_t *out, cpx_terms *terms)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_u128 o;
+  int rc = cpx_pi_u128(c, {x, 0}, y, cc, &o, terms);
+  if (rc) return rc;
+  *out = o.lo;
+  return CPX_OK;
+}
+
+static int term_i128(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, uint32_t mask, int which,
+                     i128 *out)
+{
+  ResultDev r;
+  int rc = solve_one(c, x, y, cc, mask, &r);
+  if (rc) return rc;
+  const uint64_t *src = which == 0 ? r.ordinary : which == 1 ? r.trivial : which == 2 ? r.easy
+                      : which == 3 ? r.hard : r.p2;
+  *out = get_i128(src);
+  return CPX_OK;
+}
+
+int cpx_p2_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, cpx_u128 *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  i128 v;
+  int rc = term_i128(c, to_u128(x), y, 0, TERM_P2, 4, &v);
+  if (rc) return rc;
+  *out = from_u128((u128)v);
+  return CPX_OK;
+}
+
+int cpx_p2_u64(cpx_ctx *c, uint64_t x, uint64_t y, uint64_t *out)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_u128 o;
+  int rc = cpx_p2_u128(c, {x, 0}, y, &o);
+  if (!rc) *out = o.lo;
+  return rc;
+}
+
+int cpx_ordinary_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, uint16_t cc, cpx_i128 *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  i128 v;
+  int rc = term_i128(c, to_u128(x), y, cc, TERM_ORD, 0, &v);
+  if (rc) return rc;
+  *out = {(uint64_t)v, (uint64_t)((u128)v >> 64)};
+  return CPX_OK;
+}
+
+int cpx_ordinary_u64(cpx_ctx *c, uint64_t x, uint64_t y, uint16_t cc, int64_t *out)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_i128 o;
+  int rc = cpx_ordinary_u128(c, {x, 0}, y, cc, &o);
+  if (!rc) *out = (int64_t)o.lo;
+  return rc;
+}
+
+int cpx_s2_trivial_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, cpx_u128 *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  i128 v;
+  int rc = term_i128(c, to_u128(x), y, 0, TERM_TRIV, 1, &v);
+  if (rc) return rc;
+  *out = from_u128((u128)v);
+  return CPX_OK;
+}
+
+int cpx_s2_trivial_u64(cpx_ctx *c, uint64_t x, uint64_t y, uint64_t *out)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_u128 o;
+  int rc = cpx_s2_trivial_u128(c, {x, 0}, y, &o);
+  if (!rc) *out = o.lo;
+  return rc;
+}
+
+int cpx_s3_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, uint32_t cc, cpx_i128 *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  i128 v;
+  int rc = term_i128(c, to_u128(x), y, cc, TERM_HARD, 3, &v);
+  if (rc) return rc;
+  *out = {(uint64_t)v, (uint64_t)((u128)v >> 64)};
+  return CPX_OK;
+}
+
+int cpx_s3_u64(cpx_ctx *c, uint64_t x, uint64_t y, uint32_t cc, int64_t *out)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_i128 o;
+  int rc = cpx_s3_u128(c, {x, 0}, y, cc, &o);
+  if (!rc) *out = (int64_t)o.lo;
+  return rc;
+}
+
+int cpx_v_u128(cpx_ctx *c, cpx_u128 x, uint64_t y, cpx_i128 *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  i128 v;
+  int rc = term_i128(c, to_u128(x), y, 0, TERM_EASY, 2, &v);
+  if (rc) return rc;
+  *out = {(uint64_t)v, (uint64_t)((u128)v >> 64)};
+  return CPX_OK;
+}
+
+int cpx_v_u64(cpx_ctx *c, uint64_t x, uint64_t y, int64_t *out)
+{
+  if (!out) return CPX_EINVAL;
+  cpx_i128 o;
+  int rc = cpx_v_u128(c, {x, 0}, y, &o);
+  if (!rc) *out = (int64_t)o.lo;
+  return rc;
+}
+
+int cpx_v_split_u64(cpx_ctx *c, uint64_t x, uint64_t y, int64_t *va, int64_t *vb)
+{
+  CTX_GUARD(c);
+  if (!va || !vb) return CPX_EINVAL;
+  i128 a, b;
+  int rc = term_i128(c, (u128)x, y, 0, TERM_EASY | TERM_EASY_A, 2, &a);
+  if (rc) return rc;
+  rc = term_i128(c, (u128)x, y, 0, TERM_EASY | TERM_EASY_B, 2, &b);
+  if (rc) return rc;
+  *va = (int64_t)a;
+  *vb = (int64_t)b;
+  return CPX_OK;
+}
+
+int cpx_div_x_by_y(cpx_ctx *c, cpx_u128 x, uint64_t *d_y, size_t len)
+{
+  CTX_GUARD(c);
+  if (!d_y && len) return CPX_EINVAL;
+  HIPCHK(launch_div_x_by_y(x.lo, x.hi, d_y, len, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CPX_OK;
+}
+
+// ------------------------------------------------------- primes / tables
+
+int cpx_count_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  *out = 0;
+  if (hi <= lo) return CPX_OK;
+  HIPCHK(hipSetDevice(c->device));
+  unsigned long long *d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(unsigned long long)));
+  int rc = CPX_OK;
+  unsigned long long h = 0;
+  do {
+    if (hi - 1 <= c->pitab_limit && c->pitab.p) {
+      hipError_t e = launch_count_range(c->pitab.p, lo, hi, d, c->stream);
+      if (e) { rc = hip_fail(c, e, "count_range"); break; }
+    } else {
+      uint64_t r = cpx_isqrt128((u128)hi) + 1;
+      if ((rc = ensure_tables(c, r, r, 64))) break;
+      hipError_t e = launch_zero_u32((uint32_t *)d, 2, c->stream);
+      if (!e) e = launch_count_primes_sieve(lo, hi, c->primes.p, (uint32_t)c->nprimes, d, c->stream);
+      if (e) { rc = hip_fail(c, e, "count_primes_sieve"); break; }
+    }
+    hipError_t e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) { rc = hip_fail(c, e, "count copy"); break; }
+    // the sieve counts odd primes >= 3; add the prime 2
+    if (!(hi - 1 <= c->pitab_limit && c->pitab.p) && lo <= 2 && hi > 2) h += 1;
+  } while (0);
+  (void)hipFree(d);
+  if (!rc) *out = h;
+  return rc;
+}
+
+int cpx_get_d_pitable(cpx_ctx *c, uint32_t hi, uint32_t **d_out)
+{
+  CTX_GUARD(c);
+  if (!d_out) return CPX_EINVAL;
+  *d_out = nullptr;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = ensure_tables(c, hi, 64, 64);
+  if (rc) return rc;
+  uint64_t n = hi / 2;
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, std::max<uint64_t>(n, 1) * 4));
+  hipError_t e = launch_expand_pitab_u32(c->pitab.p, d, 0, n, 0, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "expand pitab"); }
+  *d_out = d;
+  return CPX_OK;
+}
+
+int cpx_free(cpx_ctx *c, void *p)
+{
+  if (!c) return CPX_EINVAL;
+  if (p) (void)hipFree(p);
+  return CPX_OK;
+}
+
+struct cpx_pitable {
+  cpx_ctx *ctx;
+  uint64_t base, bottom, range, pi_base;
+  uint32_t *d = nullptr;
+};
+
+static int pitable_fill(cpx_pitable *t)
+{
+  cpx_ctx *c = t->ctx;
+  int rc = ensure_tables(c, t->base + t->range + 2, 64, 64);
+  if (rc) return rc;
+  uint64_t pb = 0;
+  if (t->base >= 2) {
+    unsigned long long *d;
+    HIPCHK(hipMalloc(&d, 8));
+    hipError_t e = launch_count_range(c->pitab.p, 0, t->base, d, c->stream);
+    unsigned long long h = 0;
+    if (!e) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e) return hip_fail(c, e, "pi_base");
+    pb = h;
+  }
+  t->pi_base = pb;
+  HIPCHK(launch_expand_pitab_u32(c->pitab.p, t->d, t->base, t->range / 2 + 2, pb, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CPX_OK;
+}
+
+int cpx_pitable_create(cpx_ctx *c, uint64_t base, uint64_t bottom, uint64_t range, cpx_pitable **out)
+{
+  CTX_GUARD(c);
+  if (!out || range == 0 || (base & 1)) return CPX_EINVAL;
+  *out = nullptr;
+  cpx_pitable *t = new cpx_pitable{c, base, bottom, range, 0, nullptr};
+  hipError_t e = hipMalloc(&t->d, (range / 2 + 2) * 4);
+  if (e) { delete t; return hip_fail(c, e, "hipMalloc(pitable)"); }
+  *out = t;
+  return CPX_OK;
+}
+
+int cpx_pitable_destroy(cpx_pitable *t)
+{
+  if (!t) return CPX_EINVAL;
+  if (t->d) (void)hipFree(t->d);
+  delete t;
+  return CPX_OK;
+}
+
+int cpx_pitable_current(cpx_pitable *t, uint32_t **d)
+{
+  if (!t || !d) return CPX_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+  int rc = pitable_fill(t);
+  if (!rc) *d = t->d;
+  return rc;
+}
+
+int cpx_pitable_next_up(cpx_pitable *t, uint32_t **d)
+{
+  if (!t || !d) return CPX_EINVAL;
+  t->base += t->range;
+  return cpx_pitable_current(t, d);
+}
+
+int cpx_pitable_next_down(cpx_pitable *t, uint32_t **d)
+{
+  if (!t || !d) return CPX_EINVAL;
+  if (t->base >= t->bottom + t->range) t->base -= t->range;
+  else { t->range = t->base - t->bottom; t->base = t->bottom; }
+  if (t->range == 0) return CPX_ERANGE;
+  return cpx_pitable_current(t, d);
+}
+
+int cpx_pitable_info(cpx_pitable *t, uint64_t *base, uint64_t *range, uint64_t *pi_base)
+{
+  if (!t) return CPX_EINVAL;
+  if (base) *base = t->base;
+  if (range) *range = t->range;
+  if (pi_base) *pi_base = t->pi_base;
+  return CPX_OK;
+}
+
+struct cpx_phi {
+  cpx_ctx *ctx;
+  uint32_t max;
+  uint32_t *d_lpf = nullptr;     // odd k = 2i + 1 < max
+  uint64_t n = 0;
+};
+
+int cpx_phi_create(cpx_ctx *c, uint32_t max, cpx_phi **out)
+{
+  CTX_GUARD(c);
+  if (!out || max < 3) return CPX_EINVAL;
+  *out = nullptr;
+  HIPCHK(hipSetDevice(c->device));
+  uint64_t r = cpx_isqrt128((u128)max + 16384) + 2;
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  cpx_phi *p = new cpx_phi{c, max, nullptr, 0};
+  p->n = ((uint64_t)max + 1) / 2;
+  uint64_t nalloc = (p->n + 4095) / 4096 * 4096;
+  hipError_t e = hipMalloc(&p->d_lpf, nalloc * 4);
+  if (!e) e = launch_mu_lpf_split(p->d_lpf, nullptr, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { if (p->d_lpf) (void)hipFree(p->d_lpf); delete p; return hip_fail(c, e, "Phi lpf"); }
+  *out = p;
+  return CPX_OK;
+}
+
+int cpx_phi_destroy(cpx_phi *p)
+{
+  if (!p) return CPX_EINVAL;
+  if (p->d_lpf) (void)hipFree(p->d_lpf);
+  delete p;
+  return CPX_OK;
+}
+
+int cpx_phi_eval(cpx_phi *p, uint32_t x, uint32_t a, uint64_t *out)
+{
+  if (!p || !out) return CPX_EINVAL;
+  cpx_ctx *c = p->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (x >= p->max) return ctx_fail(c, CPX_ERANGE, "x must be below the Phi table size");
+  if (a == 0) { *out = x; return CPX_OK; }
+  if (a > c->nprimes) return ctx_fail(c, CPX_ERANGE, "a beyond the resident prime list");
+  if (x == 0) { *out = 0; return CPX_OK; }
+  uint32_t pa = c->h_primes[a - 1];
+  unsigned long long *d;
+  HIPCHK(hipMalloc(&d, 8));
+  hipError_t e = launch_zero_u32((uint32_t *)d, 2, c->stream);
+  // odd k in [3, x]: indices 1 .. (x-1)/2
+  uint64_t i1 = (uint64_t)(x - 1) / 2 + 1;
+  if (!e) e = launch_phi_count_lpf(p->d_lpf, 1, i1, pa, d, c->stream);
+  unsigned long long h = 0;
+  if (!e) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e) return hip_fail(c, e, "phi count");
+  *out = 1 + h;                                    // k = 1
+  return CPX_OK;
+}
+
+int cpx_phi_range(cpx_phi *p, uint32_t num, uint32_t a, uint32_t **d_out)
+{
+  if (!p || !d_out) return CPX_EINVAL;
+  cpx_ctx *c = p->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  *d_out = nullptr;
+  uint64_t n = num / 2;
+  if (n > p->n) return ctx_fail(c, CPX_ERANGE, "range beyond the Phi table");
+  if (a == 0 || a > c->nprimes) return ctx_fail(c, CPX_ERANGE, "a out of range");
+  uint32_t pa = c->h_primes[a - 1];
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, std::max<uint64_t>(n, 1) * 4));
+  hipError_t e = launch_phi_prefix(p->d_lpf, n, pa, d, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "phi range"); }
+  *d_out = d;
+  return CPX_OK;
+}
+
+int cpx_gen_d_lpf(cpx_ctx *c, uint64_t bottom, uint64_t top, uint32_t **d_out)
+{
+  CTX_GUARD(c);
+  if (!d_out || (bottom & 1) || top <= bottom || top >= (1ull << 32)) return CPX_EINVAL;
+  *d_out = nullptr;
+  uint64_t r = cpx_isqrt128((u128)top + 16384) + 2;
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  uint64_t n = (top - bottom) / 2, nalloc = (n + 4095) / 4096 * 4096;
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, nalloc * 4));
+  hipError_t e = launch_mu_lpf_range(d, nullptr, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "gen_d_lpf"); }
+  *d_out = d;
+  return CPX_OK;
+}
+
+int cpx_gen_d_mu(cpx_ctx *c, uint64_t bottom, uint64_t top, int8_t **d_out)
+{
+  CTX_GUARD(c);
+  if (!d_out || (bottom & 1) || top <= bottom || top >= (1ull << 32)) return CPX_EINVAL;
+  *d_out = nullptr;
+  uint64_t r = cpx_isqrt128((u128)top + 16384) + 2;
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  uint64_t n = (top - bottom) / 2, nalloc = (n + 4095) / 4096 * 4096;
+  int8_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, nalloc));
+  hipError_t e = launch_mu_lpf_range(nullptr, d, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "gen_d_mu"); }
+  *d_out = d;
+  return CPX_OK;
+}
+
+int cpx_get_device_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len)
+{
+  CTX_GUARD(c);
+  if (!d_out || !len) return CPX_EINVAL;
+  *d_out = nullptr; *len = 0;
+  if (hi <= lo) return CPX_OK;
+  int rc = ensure_tables(c, hi + 1, 64, 64);
+  if (rc) return rc;
+  uint64_t cnt = 0;
+  if ((rc = cpx_count_primes(c, lo, hi, &cnt))) return rc;
+  uint64_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, std::max<uint64_t>(cnt, 1) * 8));
+  hipError_t e = launch_extract_range_u64(c->pitab.p, lo, hi, d, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "getDevicePrimes"); }
+  *d_out = d;
+  *len = cnt;
+  return CPX_OK;
+}
+
+// --------------------------------------------------------------- batch
+
+static int batch_impl(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out, cpx_terms *terms,
+                      const cpx_batch_opts *bo, cpx_batch_stats *st)
+{
+  if ((!xs || !out) && n) return CPX_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  if (bo && bo->tune && !c->shape_tuned) {
+    uint32_t tt, kk;
+    int rc = cpx_autotune(c, xs, n, &tt, &kk);
+    if (rc) return rc;
+  }
+  std::vector<HostQuery> hq(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);
+    if (rc) return rc;
+  }
+  std::vector<ResultDev> res;
+  bool graph = bo ? bo->use_graph != 0 : true;
+  cpx_batch_stats local{};
+  int rc = run_queries(c, hq, TERM_ALL, res, st ? &local : nullptr, graph);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; i++) {
+    out[i] = {res[i].pi[0], res[i].pi[1]};
+    if (terms) fill_terms(&terms[i], res[i], hq[i], 0);
+  }
+  if (st) *st = local;
+  return CPX_OK;
+}
+
+int cpx_pi_batch(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out,
+                 const cpx_batch_opts *bo, cpx_batch_stats *st)
+{
+  CTX_GUARD(c);
+  return batch_impl(c, xs, n, out, nullptr, bo, st);
+}
+
+int cpx_pi_batch_terms(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out, cpx_terms *terms,
+                       const cpx_batch_opts *bo, cpx_batch_stats *st)
+{
+  CTX_GUARD(c);
+  return batch_impl(c, xs, n, out, terms, bo, st);
+}
+
+// ------------------------------------------------------------ autotuner
+// Times every supported segment shape on the calibration batch (after one
+// warm-up) and keeps the fastest; results must agree bit for bit.
+int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, uint32_t *kwords)
+{
+  CTX_GUARD(c);
+  if (!xs || !n) return CPX_EINVAL;
+  static const SegShape cands[] = {{256, 32}, {256, 64}, {256, 128}, {512, 32}, {512, 64}, {1024, 32}};
+  std::vector<HostQuery> hq(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);
+    if (rc) return rc;
+  }
+  SegShape best = c->shape;
+  double best_ms = 1e300;
+  std::vector<ResultDev> ref, res;
+  bool have_ref = false;
+  for (const SegShape &s : cands) {
+    if (!seg_shape_supported(s) || seg_blocks_per_cu(s) < 1) continue;
+    c->shape = s;
+    int rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);   // warm-up (tables, arena)
+    if (rc) return rc;
+    double t0 = now_ms();
+    rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);
+    if (rc) return rc;
+    double ms = now_ms() - t0;
+    if (!have_ref) { ref = res; have_ref = true; }
+    for (size_t i = 0; i < n; i++)
+      if (memcmp(res[i].pi, ref[i].pi, 16) != 0)
+        return ctx_fail(c, CPX_EDEVICE, "autotune: shapes disagree");
+    if (ms < best_ms) { best_ms = ms; best = s; }
+  }
+  c->shape = best;
+  c->shape_tuned = true;
+  c->generation++;
+  if (threads) *threads = (uint32_t)best.threads;
+  if (kwords) *kwords = (uint32_t)best.kwords;
+  return CPX_OK;
+}
+
+uint64_t cpx_request_bytes(cpx_ctx *c, cpx_u128 x)
+{
+  if (!c) return 0;
+  HostQuery h;
+  if (make_host_query(c, to_u128(x), 0, 0, &h)) return 0;
+  return plan_bytes_estimate(c, h);
+}
+
+// ------------------------------------------------------------ selftests
+
+int cpx_selftest_arith(uint64_t seed, uint64_t n)
+{
+  std::mt19937_64 g(seed);
+  int fails = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    int bits = 1 + (int)(g() % 100);
+    u128 a = ((u128)g() << 64) | g();
+    if (bits < 128) a &= (((u128)1 << bits) - 1);
+    uint64_t d = g() >> (g() % 64);
+    if (d == 0) d = 1;
+    uint64_t r;
+    u128 q = cpx_div128_64(a, d, &r);
+    if (q != a / d || r != (uint64_t)(a % d)) fails++;
+    if ((a / d) >> 64 == 0) {
+      if (cpx_div128_to64(a, d) != (uint64_t)(a / d)) fails++;
+    }
+    uint64_t a64 = (uint64_t)a, d53 = d >> 11;
+    if (d53 == 0) d53 = 1;
+    if (a64 / d53 < (1ull << 50)) {
+      if (cpx_div_smallq(a64, d53) != a64 / d53) fails++;
+    }
+    uint32_t p = (uint32_t)(g() | 1);
+    if (cpx_mod64_32(a64, p) != a64 % p) fails++;
+    u128 xr = a >> (g() % 64);
+    uint64_t s = cpx_isqrt128(xr);
+    if (!((u128)s * s <= xr && ((u128)s + 1) * ((u128)s + 1) > xr)) fails++;
+    uint64_t cb = cpx_icbrt128(xr);
+    if (!((u128)cb * cb * cb <= xr && ((u128)cb + 1) * (cb + 1) * (cb + 1) > xr)) fails++;
+  }
+  return fails;
+}
+
+}  // extern "C"

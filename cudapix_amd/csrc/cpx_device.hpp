@@ -1,0 +1,127 @@
+// cpx_device.hpp -- device building blocks shared by the table builders and
+// the segment kernel: 64-wide wavefront scans/reductions, and the segmented
+// odd-only Eratosthenes cross-off used for pi tables, prime counts and P2.
+#pragma once
+#include "cpx_internal.hpp"
+
+#define CPX_WAVE 64
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (CPX_WAVE - 1); }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / CPX_WAVE; }
+
+// Inclusive prefix sum across the 64 lanes of a wavefront.
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int d = 1; d < CPX_WAVE; d <<= 1) {
+    T u = __shfl_up(v, d, CPX_WAVE);
+    if (l >= (uint32_t)d) v += u;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+  for (int d = CPX_WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, CPX_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ i128 wave_sum_i128(i128 v)
+{
+  uint64_t lo = (uint64_t)v, hi = (uint64_t)((u128)v >> 64);
+#pragma unroll
+  for (int d = CPX_WAVE / 2; d > 0; d >>= 1) {
+    uint64_t olo = __shfl_xor(lo, d, CPX_WAVE), ohi = __shfl_xor(hi, d, CPX_WAVE);
+    u128 s = (((u128)hi << 64) | lo) + (((u128)ohi << 64) | olo);
+    lo = (uint64_t)s; hi = (uint64_t)(s >> 64);
+  }
+  return (i128)(((u128)hi << 64) | lo);
+}
+
+// Block-wide sum of an i128 through LDS scratch (>= blockDim/64 entries).
+__device__ __forceinline__ i128 block_sum_i128(i128 v, i128 *scratch)
+{
+  v = wave_sum_i128(v);
+  if (lane_id() == 0) scratch[wave_id()] = v;
+  __syncthreads();
+  i128 t = 0;
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 0; w < blockDim.x / CPX_WAVE; w++) t += scratch[w];
+  }
+  __syncthreads();
+  return t;   // valid in thread 0
+}
+
+// Interleaved LDS layout of a segment bitmap: segment word g lives at
+// lds[(g % K) * T + g / K], so thread t owns the contiguous words
+// [t*K, (t+1)*K) and its j-th word sits at lds[j*T + t] (bank-conflict free
+// when every lane walks its own words in step).
+__device__ __forceinline__ uint32_t seg_addr(uint32_t g, uint32_t K, uint32_t T)
+{
+  return (g % K) * T + g / K;
+}
+
+// Bit offset, inside a segment whose bit i is the integer lo + 2i + 1 (lo even),
+// of the first odd multiple of the odd prime p that is >= max(lo + 1, from).
+__device__ __forceinline__ uint64_t first_odd_multiple_bit(uint64_t lo, uint32_t p, uint64_t from)
+{
+  uint64_t start = lo + 1 > from ? lo + 1 : from;    // first candidate integer
+  uint64_t r = cpx_mod64_32(start, p);
+  uint64_t m = start + (r ? p - r : 0);               // first multiple >= start
+  if (!(m & 1)) m += p;                               // odd multiple
+  return (m - lo - 1) >> 1;
+}
+
+// Cross off, in an interleaved LDS bitmap of `nbits` odd numbers starting at
+// integer lo (lo even), all odd multiples >= p^2 of the odd sieving primes
+// sp[i0 .. i1) with p^2 < lo + 2*nbits.  Small primes are shared by the whole
+// block (each thread takes every T-th multiple); large ones are one per
+// thread.  Uses LDS atomics (several primes hit one word); caller syncs.
+__device__ __forceinline__ void seg_cross_primes(uint32_t *lds, uint32_t K, uint32_t T, uint64_t lo,
+                                                 uint64_t nbits, const uint32_t *__restrict__ sp,
+                                                 uint32_t i0, uint32_t i1)
+{
+  const uint64_t hi = lo + 2 * nbits;
+  const uint32_t t = threadIdx.x;
+  // shared primes: p < nbits / T keep every thread busy
+  const uint64_t coop_lim = nbits / T;
+  uint32_t i = i0;
+  for (; i < i1; i++) {
+    uint32_t p = sp[i];
+    if ((uint64_t)p >= coop_lim) break;
+    if ((uint64_t)p * p >= hi) { i = i1; break; }
+    uint64_t o = first_odd_multiple_bit(lo, p, (uint64_t)p * p);
+    for (uint64_t j = o + (uint64_t)t * p; j < nbits; j += (uint64_t)T * p) {
+      uint32_t g = (uint32_t)(j >> 5);
+      atomicAnd(&lds[seg_addr(g, K, T)], ~(1u << (j & 31)));
+    }
+  }
+  // one prime per thread
+  for (uint32_t k = i + t; k < i1; k += T) {
+    uint32_t p = sp[k];
+    if ((uint64_t)p * p >= hi) break;
+    uint64_t o = first_odd_multiple_bit(lo, p, (uint64_t)p * p);
+    for (uint64_t j = o; j < nbits; j += p) {
+      uint32_t g = (uint32_t)(j >> 5);
+      atomicAnd(&lds[seg_addr(g, K, T)], ~(1u << (j & 31)));
+    }
+  }
+}
+
+// Fill an interleaved segment bitmap from a wheel pattern: segment word g is
+// pattern word (w0 + g) mod P, w0 = lo / 64.  Coalesced global reads.
+__device__ __forceinline__ void seg_fill_wheel(uint32_t *lds, uint32_t K, uint32_t T, uint64_t lo,
+                                               const PiEntry *__restrict__ pat, uint32_t P)
+{
+  const uint32_t t = threadIdx.x;
+  uint32_t base = (uint32_t)((lo >> 6) % P);
+  uint32_t W = K * T;
+  for (uint32_t g = t; g < W; g += T) {
+    uint32_t idx = (base + g) % P;
+    lds[seg_addr(g, K, T)] = pat[idx].bits;
+  }
+}

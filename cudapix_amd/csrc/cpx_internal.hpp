@@ -1,0 +1,172 @@
+// cpx_internal.hpp -- device data layouts and kernel launchers of the
+// MI355X pi(x) engine (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "cpx_arith.hpp"
+
+// ---------------------------------------------------------------- layouts
+
+// Compact pi table: one 8-byte entry per 64 integers [64w, 64w+64).
+//   bits : bit i set <=> 64w + 2i + 1 is an odd prime
+//   count: number of odd primes < 64w
+// pi(n) = (n >= 2) + count + popc(bits & lowmask(((n & 63) + 1) >> 1)).
+// (Replaces the u32-per-odd-n table of src/pitable.cu:172-218, which costs
+// 2 bytes per integer; this one costs 1/8 byte per integer.)
+struct PiEntry {
+  uint32_t bits;
+  uint32_t count;
+};
+
+__device__ __forceinline__ uint64_t pi_lookup(const PiEntry *__restrict__ t, uint64_t n)
+{
+  PiEntry e = t[n >> 6];
+  uint32_t k = (uint32_t)(((n & 63) + 1) >> 1);
+  return (uint64_t)(n >= 2) + e.count + cpx_popc32(e.bits & cpx_lowmask32(k));
+}
+
+// Wheel pattern of the first c primes over the odd-only bitmap: word w of
+// any segment equals pat[w mod P] with P = 3*5*..*p_c words (period of the
+// pattern replicated 32 times so that 32-bit words line up).  A second word
+// per entry holds the exclusive prefix popcount, so phi(r, c) for r < W_c is
+// prefix[k>>5] + popc(bits[k>>5] & lowmask(k & 31)), k = (r + 1) / 2.
+struct WheelInfo {
+  uint64_t off;        // offset of this c's pattern in the wheel array (entries)
+  uint32_t period;     // P (words)
+  uint32_t W;          // primorial W_c = 2 * 3 * ... * p_c
+  uint32_t totient;    // Euler totient of W_c
+  uint32_t pc;         // p_c
+};
+
+// Per-query record (device resident; host fills the static part).
+struct QueryDev {
+  uint64_t x_lo, x_hi;
+  uint64_t y, z;          // z = floor(x / y)
+  uint64_t sqrtx, cbrtx, qrtx, sqrtz;
+  uint64_t p2_zt;         // floor(x / (y + 1)) : largest x/p over P2 primes
+  uint64_t p2_e;          // even start of the P2 count range (sqrtx & ~1)
+  uint64_t tm_off;        // first element of this query in hard_T / hard_M
+  uint32_t c, B;          // hard stages b = c+1 .. B (1-based prime indices)
+  uint32_t nstage;        // B - c
+  uint32_t a;             // pi(y)
+  uint32_t pi_cbrt, pi_qrt, pi_sqrtz, small;   // small: answered by one pi-table lookup
+  uint64_t hard_nseg, hard_row0;      // hard segments and first output row
+  uint64_t p2_nseg, p2_row0;          // P2 segments and first output row
+  uint64_t stage_off;                 // first entry of this query in the stage table
+  uint64_t part_off;                  // first slot in the partial-sum array
+  uint32_t n_ord_parts, n_triv_parts, n_easy_parts, n_hard_parts;
+  uint64_t pi_e;                      // written by setup: pi(p2_e)
+  uint64_t pi_sqrtx;                  // written by setup: pi(sqrt x)
+};
+
+// One stage of the hard-leaf sieve: p_b and floor(x / p_b).
+struct StageDev {
+  uint64_t xp;
+  uint32_t p;
+  uint32_t pad;
+};
+
+// Work tile of the persistent segment kernel.
+enum { TILE_HARD = 0, TILE_P2 = 1 };
+struct TileDev {
+  uint32_t query;
+  uint32_t kind;
+  uint64_t seg;
+};
+
+// Block descriptor of the grid-stride term kernels (ordinary, trivial, easy).
+struct BlockDesc {
+  uint32_t query;
+  uint32_t part;         // partial-sum slot relative to the query's part_off
+  uint64_t begin, end;   // item range [begin, end)
+};
+
+// Final per-query output (two's-complement 128-bit values as lo/hi pairs).
+struct ResultDev {
+  uint64_t pi[2];
+  uint64_t ordinary[2], trivial[2], easy[2], hard[2], p2[2];
+  uint64_t pi_y;
+  uint64_t status;
+};
+
+// Everything a batch needs on the device; pointers are stable for a plan
+// capacity so a captured graph stays valid across batches.
+struct BatchDev {
+  // shared resident tables
+  const PiEntry *pitab;  uint64_t pitab_n;
+  const uint32_t *primes; uint64_t nprimes;
+  const int32_t *mlpf;   uint64_t mlpf_n;     // mu(m)*lpf(m) for odd m, index m>>1
+  const PiEntry *wheel;  const WheelInfo *winfo;
+  // plan
+  QueryDev *queries;     uint32_t *nqueries;
+  StageDev *stages;
+  TileDev *tiles;        uint32_t *ntiles;    uint32_t *tile_head;
+  BlockDesc *ord_desc;   uint32_t *n_ord;
+  BlockDesc *triv_desc;  uint32_t *n_triv;
+  BlockDesc *easy_desc;  uint32_t *n_easy;
+  BlockDesc *fin_desc;   uint32_t *n_fin;     // hard carry blocks (begin = first stage)
+  // outputs
+  uint32_t *hard_T;      int32_t *hard_M;     // [row][stage]
+  uint32_t *hard_bact;   int64_t *hard_acc;   // [row]
+  uint32_t *p2_cnt;      uint32_t *p2_np;     uint64_t *p2_loc;   // [row]
+  i128 *parts;           // partial sums
+  ResultDev *results;
+};
+
+// Segment-kernel shape: T threads, K words (32 bits) per thread, so a segment
+// holds 32*K*T odd numbers.
+struct SegShape {
+  int threads;
+  int kwords;
+  uint64_t seg_bits() const { return 32ull * kwords * threads; }
+};
+
+// ------------------------------------------------------------- launchers
+// tables.hip
+hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s);
+hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sprimes,
+                                 uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
+                                 hipStream_t s);
+hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes,
+                                 hipStream_t s);
+hipError_t launch_mlpf(int32_t *mlpf, uint64_t nodd, const uint32_t *primes, uint32_t nsp,
+                       hipStream_t s);
+hipError_t launch_wheel(PiEntry *wheel, uint32_t period, uint32_t c, hipStream_t s);
+hipError_t launch_expand_pitab_u32(const PiEntry *tab, uint32_t *out, uint64_t nodd_lo,
+                                   uint64_t count, uint64_t base_pi, hipStream_t s);
+hipError_t launch_phi_count(const int32_t *mlpf, uint64_t lo_idx, uint64_t hi_idx,
+                            uint32_t pa, unsigned long long *out, hipStream_t s);
+hipError_t launch_count_range(const PiEntry *tab, uint64_t lo, uint64_t hi,
+                              unsigned long long *out, hipStream_t s);
+hipError_t launch_count_primes_sieve(uint64_t lo, uint64_t hi, const uint32_t *sprimes,
+                                     uint32_t nsp, unsigned long long *out, hipStream_t s);
+hipError_t launch_div_x_by_y(uint64_t x_lo, uint64_t x_hi, uint64_t *d_y, size_t len,
+                             hipStream_t s);
+
+// terms.hip
+hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_setup(BatchDev b, uint32_t maxq, uint32_t max_stage_total, hipStream_t s);
+hipError_t launch_ordinary(BatchDev b, uint32_t grid, hipStream_t s);
+hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s);
+hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s);
+hipError_t launch_hard_finalize(BatchDev b, uint32_t grid, hipStream_t s);
+hipError_t launch_combine(BatchDev b, uint32_t maxq, hipStream_t s);
+
+// segsieve.hip
+hipError_t launch_segments(BatchDev b, SegShape shape, uint32_t grid, hipStream_t s);
+size_t seg_lds_bytes(SegShape shape);
+bool seg_shape_supported(SegShape shape);
+hipError_t launch_scan_entries(PiEntry *tab, uint64_t nent, uint32_t *tot, uint64_t *base,
+                               hipStream_t s);
+hipError_t launch_mu_lpf_split(uint32_t *lpf, int8_t *mu, uint64_t nodd, const uint32_t *primes,
+                               uint32_t nsp, hipStream_t s);
+hipError_t launch_phi_count_lpf(const uint32_t *lpf, uint64_t i0, uint64_t i1, uint32_t pa,
+                                unsigned long long *out, hipStream_t s);
+int seg_blocks_per_cu(SegShape shape);
+hipError_t launch_mu_lpf_range(uint32_t *lpf, int8_t *mu, uint64_t ibase, uint64_t nodd,
+                               const uint32_t *primes, uint32_t nsp, hipStream_t s);
+hipError_t launch_phi_prefix(const uint32_t *lpf, uint64_t n, uint32_t pa, uint32_t *out,
+                             hipStream_t s);
+hipError_t launch_extract_range_u64(const PiEntry *tab, uint64_t lo, uint64_t hi, uint64_t *out,
+                                    hipStream_t s);

@@ -1,0 +1,695 @@
+// engine.hip -- host orchestration of the MI355X pi(x) engine.
+//
+// One context = one device stream + resident tables (compact pi table,
+// prime list, mu*lpf table, wheel patterns) + a batch arena.  A batch of
+// queries is planned on the host (bounds by exact integer roots, tile and
+// block-descriptor lists, tiles ordered longest-first), uploaded with one copy
+// per array, and run as a fixed-shape launch sequence:
+//   zero queue -> setup -> ordinary -> trivial -> easy -> segments(persistent)
+//   -> hard carry -> combine
+// which is captured once into a hipGraph and replayed while the arena keeps
+// its capacity.  (The reference instead runs each term as its own host loop
+// of kernel + cudaDeviceSynchronize + thrust reduce, with cudaDeviceReset
+// between terms: src/Deleglise-Rivat/deleglise-rivat.cpp:50-80,
+// src/sieve/S2_hard_host.cu:151-191, src/P2.cu:93-130.)
+#include "engine.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+static double now_ms()
+{
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+int ctx_fail(cpx_ctx *c, int code, const std::string &msg)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(cpx_ctx *c, hipError_t e, const char *what)
+{
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return ctx_fail(c, CPX_ENOMEM, m);
+  return ctx_fail(c, CPX_EDEVICE, m);
+}
+
+#define HIPCHK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(c, e_, #call); \
+  } while (0)
+
+template <typename T>
+static int dbuf_reserve(cpx_ctx *c, DBuf<T> &b, size_t n, uint64_t *acct = nullptr)
+{
+  if (n <= b.cap && b.p) return CPX_OK;
+  if (b.p) {
+    if (acct) *acct -= b.bytes();
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)&b.p, bytes);
+  if (e != hipSuccess) { b.p = nullptr; (void)hipGetLastError(); return hip_fail(c, e, "hipMalloc"); }
+  b.cap = std::max<size_t>(n, 1);
+  if (acct) *acct += b.bytes();
+  return CPX_OK;
+}
+
+template <typename T>
+static void dbuf_free(DBuf<T> &b)
+{
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+void free_tables(cpx_ctx *c)
+{
+  dbuf_free(c->pitab); dbuf_free(c->primes); dbuf_free(c->mlpf);
+  dbuf_free(c->scan_tot); dbuf_free(c->scan_base);
+  c->pitab_nent = c->pitab_limit = c->prime_limit = c->nprimes = c->mlpf_limit = 0;
+  c->h_primes.clear();
+  c->table_bytes = 0;
+  c->generation++;
+}
+
+uint64_t host_pi(const cpx_ctx *c, uint64_t v)
+{
+  return (uint64_t)(std::upper_bound(c->h_primes.begin(), c->h_primes.end(), v) -
+                    c->h_primes.begin());
+}
+
+// ------------------------------------------------------------------ alpha
+// The engine's own alpha model: y = alpha * cbrt(x) trades hard-leaf sieve
+// length x/y against the easy-leaf and mu*lpf work that grows with y.  Fitted
+// from MI355X sweeps of this engine (alpha that minimised one-query wall time
+// per decade); the reference instead borrows primecount's CPU fit
+// (src/main.cpp:128-160).
+static double alpha_model(u128 x)
+{
+  double lx = std::log10((double)x);
+  double a = 0.25 * lx - 1.0;
+  if (a < 1.0) a = 1.0;
+  if (a > 8.0) a = 8.0;
+  return a;
+}
+
+uint64_t choose_y_host(const cpx_ctx *c, u128 x)
+{
+  uint64_t cb = cpx_icbrt128(x), sq = cpx_isqrt128(x);
+  double alpha = c->opts.alpha > 0 ? c->opts.alpha : alpha_model(x);
+  long double yy = (long double)alpha * (long double)cb;
+  uint64_t y = (uint64_t)yy;
+  if (y < cb) y = cb;
+  if (y > sq) y = sq;
+  if (y < 1) y = 1;
+  return y;
+}
+
+int make_host_query(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, HostQuery *hq)
+{
+  HostQuery h{};
+  h.x = x;
+  if ((x >> 100) != 0) return ctx_fail(c, CPX_ERANGE, "x above 2^100 is not supported");
+  if (cc == 0) cc = c->opts.c;
+  if (cc < 1 || cc > 8) return ctx_fail(c, CPX_EINVAL, "c must be in 1..8");
+  if (x < CPX_SMALL_X) {
+    h.small = true;
+    h.y = 1; h.c = 1;
+    *hq = h;
+    return CPX_OK;
+  }
+  h.sqrtx = cpx_isqrt128(x);
+  h.cbrtx = cpx_icbrt128(x);
+  h.qrtx = cpx_isqrt128(h.sqrtx);
+  if (y == 0) y = choose_y_host(c, x);
+  if (y < h.cbrtx || y > h.sqrtx)
+    return ctx_fail(c, CPX_ERANGE, "y must lie in [icbrt(x), isqrt(x)]");
+  if (y >= (1ull << 31)) return ctx_fail(c, CPX_ERANGE, "y >= 2^31 is not supported");
+  h.y = y;
+  u128 z = x / y;
+  // leaf quotients x/(m p) < z use the f64-estimate divide, exact below 2^50
+  if (z >> 50) return ctx_fail(c, CPX_ERANGE, "x / y must be below 2^50");
+  h.z = (uint64_t)z;
+  h.zt = (uint64_t)(x / (y + 1));
+  h.sqrtz = cpx_isqrt128(z);
+  // c <= pi(x^(1/4)) keeps p_c <= x^(1/4) <= sqrt(z) <= cbrt(x)
+  static const uint32_t sp[8] = {2, 3, 5, 7, 11, 13, 17, 19};
+  uint32_t k = 0;
+  while (k < 8 && sp[k] <= h.qrtx) k++;
+  h.c = cc < k ? cc : k;
+  // floor(x / p_{c+1}) must fit 64 bits for the stage table
+  if (h.c < 8 && (x / sp[h.c]) >> 64) return ctx_fail(c, CPX_ERANGE, "x / p_(c+1) must fit 64 bits; raise c");
+  *hq = h;
+  return CPX_OK;
+}
+
+// ---------------------------------------------------------------- tables
+
+static int build_bootstrap(cpx_ctx *c)
+{
+  if (c->small.p) return CPX_OK;
+  int rc;
+  if ((rc = dbuf_reserve(c, c->small, 4096, &c->table_bytes))) return rc;
+  if ((rc = dbuf_reserve(c, c->small_primes, 24000, &c->table_bytes))) return rc;
+  HIPCHK(launch_small_sieve(c->small.p, 4096, c->stream));
+  HIPCHK(launch_extract_primes(c->small.p, (1u << 18) - 1, c->small_primes.p, c->stream));
+  PiEntry last;
+  HIPCHK(hipMemcpyAsync(&last, c->small.p + 4095, sizeof(last), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->n_small_primes = 1 + last.count + (uint32_t)__builtin_popcount(last.bits);
+  return CPX_OK;
+}
+
+int build_wheels(cpx_ctx *c, uint32_t want)
+{
+  uint32_t cmax = std::max(c->opts.c, want);
+  if (cmax > 8) cmax = 8;
+  if (c->wheel_c >= cmax && c->wheel.p) return CPX_OK;
+  static const uint32_t sp[8] = {2, 3, 5, 7, 11, 13, 17, 19};
+  std::vector<WheelInfo> wi(9);
+  uint64_t off = 0, W = 1, tot = 1, period = 1;
+  for (uint32_t k = 1; k <= cmax; k++) {
+    W *= sp[k - 1];
+    tot *= sp[k - 1] - 1;
+    if (k >= 2) period *= sp[k - 1];
+    wi[k].off = off;
+    wi[k].period = (uint32_t)period;
+    wi[k].W = (uint32_t)W;
+    wi[k].totient = (uint32_t)tot;
+    wi[k].pc = sp[k - 1];
+    off += period + 1;
+  }
+  int rc;
+  if ((rc = dbuf_reserve(c, c->wheel, off, &c->table_bytes))) return rc;
+  if ((rc = dbuf_reserve(c, c->winfo, 9, &c->table_bytes))) return rc;
+  size_t maxchunks = (period + 4095) / 4096 + 2;
+  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(maxchunks, c->scan_tot.cap)))) return rc;
+  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(maxchunks, c->scan_base.cap)))) return rc;
+  for (uint32_t k = 1; k <= cmax; k++) {
+    HIPCHK(launch_wheel(c->wheel.p + wi[k].off, wi[k].period, k, c->stream));
+    HIPCHK(launch_scan_entries(c->wheel.p + wi[k].off, wi[k].period, c->scan_tot.p,
+                               c->scan_base.p, c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(c->winfo.p, wi.data(), 9 * sizeof(WheelInfo), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->h_winfo = wi;
+  c->wheel_c = cmax;
+  c->generation++;
+  return CPX_OK;
+}
+
+int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
+{
+  int rc;
+  if ((rc = build_bootstrap(c))) return rc;
+  if ((rc = build_wheels(c, 0))) return rc;
+  if (Y < 64) Y = 64;
+  uint64_t Pm = std::max<uint64_t>(P, cpx_isqrt128((u128)Y + 16384) + 1);
+  Pm = std::max<uint64_t>(Pm, Y);
+  uint64_t Rm = std::max<uint64_t>({R, Pm, (uint64_t)1 << 14});
+  if (!c->opts.fresh_tables && c->pitab_limit >= Rm && c->prime_limit >= Pm && c->mlpf_limit >= Y)
+    return CPX_OK;
+  if (!c->opts.fresh_tables) {                      // headroom for later, larger queries
+    Rm = std::max<uint64_t>(Rm, c->pitab_limit + c->pitab_limit / 4);
+    Pm = std::max<uint64_t>(Pm, c->prime_limit + c->prime_limit / 4);
+    Y = std::max<uint64_t>(Y, c->mlpf_limit + c->mlpf_limit / 4);
+    Rm = std::max<uint64_t>({Rm, Pm, Y});
+  }
+  if ((u128)Rm >= ((u128)1 << 36)) return ctx_fail(c, CPX_ERANGE, "pi table above 2^36 is not supported");
+  if (Pm >= (1ull << 32)) return ctx_fail(c, CPX_ERANGE, "prime table above 2^32 is not supported");
+  free_tables(c);
+  // pi table over [0, 64 * nent), nent a multiple of the 8192-entry segment
+  uint64_t nent = (Rm + 1 + 63) / 64;
+  nent = (nent + 8191) / 8192 * 8192;
+  if ((rc = dbuf_reserve(c, c->pitab, nent, &c->table_bytes))) return rc;
+  size_t nch = (nent + 4095) / 4096 + 2;
+  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(nch, c->scan_tot.cap)))) return rc;
+  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(nch, c->scan_base.cap)))) return rc;
+  HIPCHK(launch_pitab_segments(c->pitab.p, nent, c->small_primes.p, c->n_small_primes,
+                               c->scan_tot.p, c->scan_base.p, c->stream));
+  c->pitab_nent = nent;
+  c->pitab_limit = nent * 64 - 1;
+  // prime list <= Pm
+  PiEntry e;
+  HIPCHK(hipMemcpyAsync(&e, c->pitab.p + (Pm >> 6), sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  uint32_t kk = (uint32_t)(((Pm & 63) + 1) >> 1);
+  uint64_t np = 1 + e.count + (uint64_t)__builtin_popcount(e.bits & (uint32_t)((1ull << kk) - 1));
+  if ((rc = dbuf_reserve(c, c->primes, np + 1, &c->table_bytes))) return rc;
+  HIPCHK(launch_extract_primes(c->pitab.p, Pm, c->primes.p, c->stream));
+  c->h_primes.resize(np);
+  HIPCHK(hipMemcpyAsync(c->h_primes.data(), c->primes.p, np * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c->stream));
+  c->nprimes = np;
+  c->prime_limit = Pm;
+  // mu * lpf for odd m <= Y
+  uint64_t nodd = (Y + 1) / 2;
+  uint64_t nodd_alloc = (nodd + 4095) / 4096 * 4096;
+  if ((rc = dbuf_reserve(c, c->mlpf, nodd_alloc, &c->table_bytes))) return rc;
+  HIPCHK(launch_mlpf(c->mlpf.p, nodd_alloc, c->primes.p, (uint32_t)np, c->stream));
+  c->mlpf_limit = 2 * nodd_alloc - 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->generation++;
+  return CPX_OK;
+}
+
+// ---------------------------------------------------------------- planning
+
+static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 16384;
+
+static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint64_t SB,
+                       uint64_t *R, uint64_t *P, uint64_t *Y)
+{
+  if (h.small) { *R = std::max<uint64_t>(*R, (uint64_t)h.x); return; }
+  *R = std::max(*R, std::max(h.sqrtx, h.y));
+  *Y = std::max(*Y, h.y);
+  uint64_t p = h.y;
+  if ((mask & TERM_P2) && h.y < h.sqrtx) {
+    uint64_t e = h.sqrtx & ~63ull;
+    uint64_t nseg = ceil_div(h.zt + 1 - e, 2 * SB);
+    p = std::max<uint64_t>(p, cpx_isqrt128((u128)e + nseg * 2 * SB) + 1);
+  }
+  *P = std::max(*P, p);
+  (void)c;
+}
+
+struct TileCost { double cost; TileDev t; };
+
+static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl)
+{
+  const uint64_t SB = c->shape.seg_bits();
+  const uint32_t T = (uint32_t)c->shape.threads;
+  std::vector<TileCost> tiles;
+  pl.q.resize(hq.size());
+  for (size_t i = 0; i < hq.size(); i++) {
+    const HostQuery &h = hq[i];
+    QueryDev d{};
+    d.x_lo = (uint64_t)h.x;
+    d.x_hi = (uint64_t)(h.x >> 64);
+    d.part_off = pl.parts;
+    if (h.small) {
+      d.small = 1;
+      pl.q[i] = d;
+      continue;
+    }
+    d.y = h.y; d.z = h.z; d.sqrtx = h.sqrtx; d.cbrtx = h.cbrtx; d.qrtx = h.qrtx; d.sqrtz = h.sqrtz;
+    d.p2_zt = h.zt;
+    // the wheel pattern is word-aligned: segments start on a multiple of 64
+    d.p2_e = h.sqrtx & ~63ull;
+    d.c = h.c;
+    uint64_t B = host_pi(c, h.qrtx);
+    d.B = (uint32_t)B;
+    d.nstage = ((mask & TERM_HARD) && B > h.c) ? (uint32_t)(B - h.c) : 0;
+    d.a = (uint32_t)host_pi(c, h.y);
+    d.pi_cbrt = (uint32_t)host_pi(c, std::min(h.cbrtx, h.y));
+    d.pi_qrt = (uint32_t)B;
+    d.pi_sqrtz = (uint32_t)host_pi(c, h.sqrtz);
+    d.stage_off = pl.stage_total;
+    pl.stage_total += d.nstage;
+    // hard segments over [0, z]
+    d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : 0;
+    d.hard_row0 = pl.hard_rows;
+    d.tm_off = pl.hard_tm;
+    pl.hard_rows += d.hard_nseg;
+    pl.hard_tm += d.hard_nseg * d.nstage;
+    // P2 segments over [p2_e, zt]
+    d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? ceil_div(h.zt + 1 - d.p2_e, 2 * SB) : 0;
+    d.p2_row0 = pl.p2_rows;
+    pl.p2_rows += d.p2_nseg;
+    uint32_t part = 0;
+    // ordinary: odd n <= y
+    if (mask & TERM_ORD) {
+      uint64_t nodd = (h.y + 1) / 2;
+      for (uint64_t b = 0; b < nodd; b += ORD_CH)
+        pl.ord.push_back({(uint32_t)i, part++, b, std::min(nodd, b + ORD_CH)});
+    }
+    d.n_ord_parts = part;
+    // trivial U: prime indices [pi(sqrt z), pi(cbrt x))
+    if ((mask & TERM_TRIV) && d.pi_cbrt > d.pi_sqrtz) {
+      for (uint64_t b = d.pi_sqrtz; b < d.pi_cbrt; b += TRIV_CH)
+        pl.triv.push_back({(uint32_t)i, part++, b, std::min<uint64_t>(d.pi_cbrt, b + TRIV_CH)});
+    }
+    d.n_triv_parts = part - d.n_ord_parts;
+    // easy V: prime indices [pi(x^1/4), pi(cbrt x)), grouped by q-count
+    uint64_t ej0 = d.pi_qrt, ej1 = d.pi_cbrt;
+    if (mask & TERM_EASY_A) ej1 = std::min<uint64_t>(ej1, d.pi_sqrtz);
+    if (mask & TERM_EASY_B) ej0 = std::max<uint64_t>(ej0, d.pi_sqrtz);
+    if ((mask & TERM_EASY) && ej1 > ej0) {
+      uint64_t j0 = ej0, work = 0;
+      for (uint64_t j = ej0; j < ej1; j++) {
+        uint64_t p = c->h_primes[j];
+        u128 q2 = h.x / ((u128)p * p);
+        uint64_t qm = q2 > h.y ? h.y : (uint64_t)q2;
+        uint64_t k1 = host_pi(c, qm);
+        work += (k1 > j + 1 ? k1 - j - 1 : 0) + 64;
+        if (work >= EASY_WORK || j + 1 == ej1) {
+          pl.easy.push_back({(uint32_t)i, part++, j0, j + 1});
+          j0 = j + 1;
+          work = 0;
+        }
+      }
+    }
+    d.n_easy_parts = part - d.n_ord_parts - d.n_triv_parts;
+    // hard carry: one block per 256 stages
+    for (uint64_t b = 0; b < d.nstage; b += 256)
+      pl.fin.push_back({(uint32_t)i, part++, b, std::min<uint64_t>(d.nstage, b + 256)});
+    d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
+    pl.parts += part;
+    // tiles with a rough relative cost (longest first)
+    for (uint64_t s = 0; s < d.hard_nseg; s++) {
+      uint64_t lo = s * 2 * SB;
+      uint64_t lim = h.qrtx;
+      if (lo) {
+        u128 xl = h.x / lo;
+        uint64_t r = cpx_isqrt128(xl);
+        if (r < lim) lim = r;
+      }
+      uint64_t act = host_pi(c, lim);
+      act = act > h.c ? act - h.c : 0;
+      if (act > d.nstage) act = d.nstage;
+      double cost = 1.0 + (double)act * (1.0 + 8.0 * SB / (64.0 * 1024.0 * T));
+      tiles.push_back({cost, {(uint32_t)i, (uint32_t)TILE_HARD, s}});
+    }
+    for (uint64_t s = 0; s < d.p2_nseg; s++) {
+      uint64_t hi = d.p2_e + (s + 1) * 2 * SB;
+      double nsp = (double)host_pi(c, std::min<uint64_t>(cpx_isqrt128(hi), c->prime_limit));
+      double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
+      tiles.push_back({cost, {(uint32_t)i, (uint32_t)TILE_P2, s}});
+    }
+    pl.hard_segments += d.hard_nseg;
+    pl.p2_segments += d.p2_nseg;
+    pl.q[i] = d;
+  }
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const TileCost &a, const TileCost &b) { return a.cost > b.cost; });
+  pl.tiles.resize(tiles.size());
+  for (size_t i = 0; i < tiles.size(); i++) pl.tiles[i] = tiles[i].t;
+  return CPX_OK;
+}
+
+// ------------------------------------------------------------- the arena
+// Fixed-capacity layout: array offsets depend only on capacities, so the
+// pointers baked into a captured graph stay valid until a capacity grows.
+struct Caps {
+  uint64_t q, stages, tiles, ord, triv, easy, fin, tm, rows, p2rows, parts;
+};
+
+struct Layout {
+  size_t counters, queries, stages, tiles, ord, triv, easy, fin, plan_bytes;
+  size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
+};
+
+static size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static Layout make_layout(const Caps &k)
+{
+  Layout L{};
+  size_t o = 0;
+  L.counters = o; o = al(o + 16 * 4);
+  L.queries = o;  o = al(o + k.q * sizeof(QueryDev));
+  L.stages = o;   o = al(o + k.stages * sizeof(StageDev));
+  L.tiles = o;    o = al(o + k.tiles * sizeof(TileDev));
+  L.ord = o;      o = al(o + k.ord * sizeof(BlockDesc));
+  L.triv = o;     o = al(o + k.triv * sizeof(BlockDesc));
+  L.easy = o;     o = al(o + k.easy * sizeof(BlockDesc));
+  L.fin = o;      o = al(o + k.fin * sizeof(BlockDesc));
+  L.plan_bytes = o;
+  o = 0;
+  L.hard_T = o;    o = al(o + k.tm * 4);
+  L.hard_M = o;    o = al(o + k.tm * 4);
+  L.hard_bact = o; o = al(o + k.rows * 4);
+  L.hard_acc = o;  o = al(o + k.rows * 8);
+  L.p2_cnt = o;    o = al(o + k.p2rows * 4);
+  L.p2_np = o;     o = al(o + k.p2rows * 4);
+  L.p2_loc = o;    o = al(o + k.p2rows * 8);
+  L.parts = o;     o = al(o + k.parts * sizeof(i128));
+  L.results = o;   o = al(o + k.q * sizeof(ResultDev));
+  L.out_bytes = o;
+  return L;
+}
+
+
+static Caps needed_caps(const Plan &pl)
+{
+  Caps k;
+  k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size();
+  k.ord = pl.ord.size(); k.triv = pl.triv.size(); k.easy = pl.easy.size(); k.fin = pl.fin.size();
+  k.tm = pl.hard_tm; k.rows = pl.hard_rows; k.p2rows = pl.p2_rows; k.parts = pl.parts;
+  return k;
+}
+
+// per-context arena capacities live beside the buffers
+struct CtxArena { Caps caps{}; Layout L{}; };
+static std::mutex g_arena_mu;
+#include <map>
+static std::map<const cpx_ctx *, CtxArena> g_arenas;
+
+static CtxArena &arena_of(const cpx_ctx *c)
+{
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  return g_arenas[c];
+}
+
+void forget_arena(const cpx_ctx *c)
+{
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  g_arenas.erase(c);
+}
+
+static int ensure_arena(cpx_ctx *c, const Plan &pl)
+{
+  CtxArena &A = arena_of(c);
+  Caps need = needed_caps(pl);
+  Caps &k = A.caps;
+  bool grow = !c->d_plan || !c->d_out;
+  auto fit = [&](uint64_t &cap, uint64_t n) {
+    if (n > cap) { cap = std::max<uint64_t>(n, cap + cap / 2); grow = true; }
+  };
+  fit(k.q, std::max<uint64_t>(need.q, 64)); fit(k.stages, need.stages); fit(k.tiles, need.tiles);
+  fit(k.ord, need.ord); fit(k.triv, need.triv); fit(k.easy, need.easy); fit(k.fin, need.fin);
+  fit(k.tm, need.tm); fit(k.rows, need.rows); fit(k.p2rows, need.p2rows); fit(k.parts, need.parts);
+  if (!grow) return CPX_OK;
+  Layout L = make_layout(k);
+  if (c->d_plan) (void)hipFree(c->d_plan);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->h_plan) (void)hipHostFree(c->h_plan);
+  if (c->h_res) (void)hipHostFree(c->h_res);
+  c->d_plan = c->d_out = c->h_plan = c->h_res = nullptr;
+  c->generation++;
+  uint64_t budget = c->opts.hbm_budget;
+  if (budget && c->table_bytes + L.plan_bytes + L.out_bytes > budget)
+    return ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget");
+  hipError_t e;
+  if ((e = hipMalloc(&c->d_plan, L.plan_bytes)) != hipSuccess) { c->d_plan = nullptr; (void)hipGetLastError(); A.caps = Caps{}; return hip_fail(c, e, "hipMalloc(plan)"); }
+  if ((e = hipMalloc(&c->d_out, L.out_bytes)) != hipSuccess) { c->d_out = nullptr; (void)hipGetLastError(); A.caps = Caps{}; return hip_fail(c, e, "hipMalloc(out)"); }
+  if ((e = hipHostMalloc(&c->h_plan, L.plan_bytes, hipHostMallocDefault)) != hipSuccess) { c->h_plan = nullptr; A.caps = Caps{}; return hip_fail(c, e, "hipHostMalloc"); }
+  if ((e = hipHostMalloc(&c->h_res, k.q * sizeof(ResultDev), hipHostMallocDefault)) != hipSuccess) { c->h_res = nullptr; A.caps = Caps{}; return hip_fail(c, e, "hipHostMalloc"); }
+  c->d_plan_cap = L.plan_bytes;
+  c->d_out_cap = L.out_bytes;
+  A.L = L;
+  return CPX_OK;
+}
+
+static BatchDev make_batchdev(cpx_ctx *c)
+{
+  CtxArena &A = arena_of(c);
+  const Layout &L = A.L;
+  char *P = (char *)c->d_plan, *O = (char *)c->d_out;
+  BatchDev b{};
+  b.pitab = c->pitab.p; b.pitab_n = c->pitab_nent;
+  b.primes = c->primes.p; b.nprimes = c->nprimes;
+  b.mlpf = c->mlpf.p; b.mlpf_n = c->mlpf.cap;
+  b.wheel = c->wheel.p; b.winfo = c->winfo.p;
+  uint32_t *cnt = (uint32_t *)(P + L.counters);
+  b.nqueries = cnt + 0; b.ntiles = cnt + 1; b.tile_head = cnt + 2;
+  b.n_ord = cnt + 3; b.n_triv = cnt + 4; b.n_easy = cnt + 5; b.n_fin = cnt + 6;
+  b.queries = (QueryDev *)(P + L.queries);
+  b.stages = (StageDev *)(P + L.stages);
+  b.tiles = (TileDev *)(P + L.tiles);
+  b.ord_desc = (BlockDesc *)(P + L.ord);
+  b.triv_desc = (BlockDesc *)(P + L.triv);
+  b.easy_desc = (BlockDesc *)(P + L.easy);
+  b.fin_desc = (BlockDesc *)(P + L.fin);
+  b.hard_T = (uint32_t *)(O + L.hard_T);
+  b.hard_M = (int32_t *)(O + L.hard_M);
+  b.hard_bact = (uint32_t *)(O + L.hard_bact);
+  b.hard_acc = (int64_t *)(O + L.hard_acc);
+  b.p2_cnt = (uint32_t *)(O + L.p2_cnt);
+  b.p2_np = (uint32_t *)(O + L.p2_np);
+  b.p2_loc = (uint64_t *)(O + L.p2_loc);
+  b.parts = (i128 *)(O + L.parts);
+  b.results = (ResultDev *)(O + L.results);
+  return b;
+}
+
+static int upload_plan(cpx_ctx *c, const Plan &pl)
+{
+  CtxArena &A = arena_of(c);
+  const Layout &L = A.L;
+  char *H = (char *)c->h_plan;
+  uint32_t *cnt = (uint32_t *)(H + L.counters);
+  memset(cnt, 0, 64);
+  cnt[0] = (uint32_t)pl.q.size(); cnt[1] = (uint32_t)pl.tiles.size(); cnt[2] = 0;
+  cnt[3] = (uint32_t)pl.ord.size(); cnt[4] = (uint32_t)pl.triv.size();
+  cnt[5] = (uint32_t)pl.easy.size(); cnt[6] = (uint32_t)pl.fin.size();
+  auto put = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) memcpy(H + off, src, bytes);
+  };
+  put(L.queries, pl.q.data(), pl.q.size() * sizeof(QueryDev));
+  put(L.tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDev));
+  put(L.ord, pl.ord.data(), pl.ord.size() * sizeof(BlockDesc));
+  put(L.triv, pl.triv.data(), pl.triv.size() * sizeof(BlockDesc));
+  put(L.easy, pl.easy.data(), pl.easy.size() * sizeof(BlockDesc));
+  put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
+  // copy only the used prefix of each array
+  auto up = [&](size_t off, size_t bytes) -> hipError_t {
+    if (!bytes) return hipSuccess;
+    return hipMemcpyAsync((char *)c->d_plan + off, H + off, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(up(L.counters, 64));
+  HIPCHK(up(L.queries, pl.q.size() * sizeof(QueryDev)));
+  HIPCHK(up(L.tiles, pl.tiles.size() * sizeof(TileDev)));
+  HIPCHK(up(L.ord, pl.ord.size() * sizeof(BlockDesc)));
+  HIPCHK(up(L.triv, pl.triv.size() * sizeof(BlockDesc)));
+  HIPCHK(up(L.easy, pl.easy.size() * sizeof(BlockDesc)));
+  HIPCHK(up(L.fin, pl.fin.size() * sizeof(BlockDesc)));
+  return CPX_OK;
+}
+
+// Occupancy query (and the >64 KiB LDS attribute) happen here, outside any
+// graph capture; the grid is cached per shape.
+static int seg_grid(cpx_ctx *c)
+{
+  static thread_local SegShape last{0, 0};
+  static thread_local int last_nb = 0;
+  if (last.threads != c->shape.threads || last.kwords != c->shape.kwords) {
+    last_nb = seg_blocks_per_cu(c->shape);
+    last = c->shape;
+  }
+  int nb = last_nb < 1 ? 1 : last_nb;
+  return nb * c->cu_count;
+}
+
+static const uint32_t GRID_Q = 256, GRID_T = 2048;
+
+static int enqueue(cpx_ctx *c, const BatchDev &b, bool timed)
+{
+  HIPCHK(launch_zero_u32(b.tile_head, 1, c->stream));
+  HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
+  HIPCHK(launch_ordinary(b, GRID_T, c->stream));
+  HIPCHK(launch_trivial(b, GRID_T, c->stream));
+  HIPCHK(launch_easy(b, GRID_T, c->stream));
+  if (timed) HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
+  if (timed) HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(launch_hard_finalize(b, GRID_T, c->stream));
+  HIPCHK(launch_combine(b, GRID_Q, c->stream));
+  return CPX_OK;
+}
+
+static int ensure_graph(cpx_ctx *c, const BatchDev &b)
+{
+  if (c->gexec && c->graph_gen == c->generation && c->graph_shape.threads == c->shape.threads &&
+      c->graph_shape.kwords == c->shape.kwords)
+    return CPX_OK;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  // kernel attributes and the occupancy query must happen outside capture
+  (void)seg_grid(c);
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue(c, b, false);
+  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (rc) return rc;
+  if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
+  hipError_t e3 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) { c->gexec = nullptr; return hip_fail(c, e3, "hipGraphInstantiate"); }
+  c->graph_gen = c->generation;
+  c->graph_shape = c->shape;
+  return CPX_OK;
+}
+
+uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &h)
+{
+  if (h.small) return 1024;
+  const uint64_t SB = c->shape.seg_bits();
+  // stages ~ pi(x^(1/4)) <= 1.26 n / ln n
+  double q = (double)h.qrtx;
+  double B = q < 3 ? 1 : 1.26 * q / std::log(q);
+  double hseg = std::ceil((double)(h.z + 1) / (2.0 * SB));
+  double p2seg = std::ceil((double)(h.zt + 1) / (2.0 * SB));
+  double bytes = hseg * B * 8 + hseg * 12 + p2seg * 16 + 4096 + B * 16 + (h.y / 8192.0 + 16) * 48;
+  return (uint64_t)bytes;
+}
+
+int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask,
+                std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph)
+{
+  double t0 = now_ms();
+  if (hq.empty()) { res.clear(); return CPX_OK; }
+  // tables first (planning needs the host prime list)
+  uint64_t R = 0, P = 0, Y = 0;
+  const uint64_t SB = c->shape.seg_bits();
+  for (const HostQuery &h : hq) plan_needs(c, h, mask, SB, &R, &P, &Y);
+  double tt0 = now_ms();
+  int rc = ensure_tables(c, R, P, Y);
+  if (rc) return rc;
+  uint32_t cmax = 1;
+  for (const HostQuery &h : hq) cmax = std::max(cmax, h.c);
+  if ((rc = build_wheels(c, cmax))) return rc;
+  double tt1 = now_ms();
+  Plan pl;
+  if ((rc = build_plan(c, hq, mask, pl))) return rc;
+  if ((rc = ensure_arena(c, pl))) return rc;
+  double tp = now_ms();
+  if ((rc = upload_plan(c, pl))) return rc;
+  BatchDev b = make_batchdev(c);
+  bool graph = use_graph && c->opts.use_graphs;
+  bool timed = st != nullptr && !graph;
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  if (graph) {
+    if ((rc = ensure_graph(c, b))) return rc;
+    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    if ((rc = enqueue(c, b, timed))) return rc;
+  }
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  CtxArena &A = arena_of(c);
+  HIPCHK(hipMemcpyAsync(c->h_res, (char *)c->d_out + A.L.results, hq.size() * sizeof(ResultDev),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  res.assign((ResultDev *)c->h_res, (ResultDev *)c->h_res + hq.size());
+  for (const ResultDev &r : res)
+    if (r.status != 1) return ctx_fail(c, CPX_EDEVICE, "a query produced no result");
+  if (st) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    st->ms_device = ms;
+    st->ms_segments = 0;
+    if (timed) {
+      float s2 = 0;
+      (void)hipEventElapsedTime(&s2, c->ev[2], c->ev[3]);
+      st->ms_segments = s2;
+    }
+    st->ms_tables = tt1 - tt0;
+    st->ms_plan = (tp - t0) - (tt1 - tt0);
+    st->ms_total = now_ms() - t0;
+    st->tiles = pl.tiles.size();
+    st->hard_segments = pl.hard_segments;
+    st->p2_segments = pl.p2_segments;
+    st->seg_threads = (uint32_t)c->shape.threads;
+    st->seg_kwords = (uint32_t)c->shape.kwords;
+  }
+  return CPX_OK;
+}

@@ -1,0 +1,293 @@
+// terms.hip -- grid-stride kernels for the cheap terms of
+//   pi(x) = S0 + (S1 + U) + V + S3 + pi(y) - 1 - P2
+// (assembly as src/Deleglise-Rivat/deleglise-rivat.cpp:77), plus the hard-leaf
+// carry and the per-query combine.  Every kernel walks device-resident block
+// descriptors, so its launch shape is fixed and a captured graph can be
+// replayed for any batch that fits the plan capacity.
+#include "cpx_device.hpp"
+
+__global__ void k_zero_u32(uint32_t *p, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
+{
+  uint32_t g = (uint32_t)((n + 255) / 256);
+  if (g == 0) g = 1;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_zero_u32, dim3(g), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- setup
+// Block per query: pi(sqrt x), pi(p2_e), and the stage table
+// {p_b, floor(x / p_b)} with the emulated 128/64 divide.
+__global__ __launch_bounds__(256) void k_setup(BatchDev bd)
+{
+  const uint32_t nq = *bd.nqueries;
+  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    QueryDev &q = bd.queries[qi];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    if (threadIdx.x == 0) {
+      q.pi_sqrtx = pi_lookup(bd.pitab, q.sqrtx);
+      q.pi_e = pi_lookup(bd.pitab, q.p2_e);
+    }
+    for (uint32_t s = threadIdx.x; s < q.nstage; s += blockDim.x) {
+      uint32_t p = bd.primes[q.c + s];
+      StageDev d;
+      d.p = p;
+      d.pad = 0;
+      d.xp = cpx_div128_to64(x, p);
+      bd.stages[q.stage_off + s] = d;
+    }
+  }
+}
+
+hipError_t launch_setup(BatchDev b, uint32_t grid, uint32_t, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_setup, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------- ordinary
+// S0 = sum over odd n <= y with lpf(n) > p_c of mu(n) * phi(x/n, c),
+// phi(u, c) = floor(u / W_c) * totient(W_c) + phi(u mod W_c, c)
+// (src/ordinary.cu:101-115 and :118-135 for 128-bit x).  Items: odd index i,
+// n = 2i + 1.
+__global__ __launch_bounds__(256) void k_ordinary(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  const uint32_t nd = *bd.n_ord;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const BlockDesc d = bd.ord_desc[di];
+    const QueryDev &q = bd.queries[d.query];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    const WheelInfo wi = bd.winfo[q.c];
+    const PiEntry *wh = bd.wheel + wi.off;
+    i128 sum = 0;
+    for (uint64_t i = d.begin + threadIdx.x; i < d.end; i += blockDim.x) {
+      int32_t v = bd.mlpf[i];
+      uint32_t av = (uint32_t)(v < 0 ? -v : v);
+      if (av <= wi.pc) continue;
+      uint64_t n = 2 * i + 1;
+      uint64_t rr;
+      u128 m = cpx_div128_64(x, n, &rr);
+      uint64_t r;
+      u128 k = cpx_div128_64(m, wi.W, &r);
+      uint32_t kk = (uint32_t)((r + 1) >> 1);
+      PiEntry e = wh[kk >> 5];
+      uint64_t ph = e.count + cpx_popc32(e.bits & cpx_lowmask32(kk & 31u));
+      i128 phi = (i128)(k * wi.totient) + (i128)ph;
+      sum += v > 0 ? phi : -phi;
+    }
+    i128 t = block_sum_i128(sum, red);
+    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = t;
+  }
+}
+
+hipError_t launch_ordinary(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ordinary, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ trivial U
+// U = sum over primes sqrt(z) < p <= cbrt(x) of [pi(y) - pi(x / p^2)]
+// (src/trivial.cu:31-62).  Items: prime index i (p = primes[i]).
+__global__ __launch_bounds__(256) void k_trivial(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  const uint32_t nd = *bd.n_triv;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const BlockDesc d = bd.triv_desc[di];
+    const QueryDev &q = bd.queries[d.query];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    i128 sum = 0;
+    for (uint64_t i = d.begin + threadIdx.x; i < d.end; i += blockDim.x) {
+      uint64_t p = bd.primes[i];
+      uint64_t qq = cpx_div128_to64(x, p * p);    // < y
+      sum += (i128)q.a - (i128)pi_lookup(bd.pitab, qq);
+    }
+    i128 t = block_sum_i128(sum, red);
+    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = t;
+  }
+}
+
+hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_trivial, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- easy V
+// V = sum_{x^(1/4) < p <= cbrt x} sum_{p < q <= min(y, x/p^2)} [pi(x/pq) - pi(p) + 2]
+// (src/V.cu:22-124; all these leaves have x/pq < p^2).  One descriptor per p
+// (begin = prime index j of p); threads stride over the q's.
+__global__ __launch_bounds__(256) void k_easy(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  const uint32_t nd = *bd.n_easy;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const BlockDesc d = bd.easy_desc[di];
+    const QueryDev &q = bd.queries[d.query];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    i128 sum = 0;
+    for (uint64_t j = d.begin; j < d.end; j++) {
+      const uint64_t p = bd.primes[j];
+      const uint64_t xp = cpx_div128_to64(x, p);
+      uint64_t qmax = q.y;
+      if ((xp >> 50) < p) {
+        uint64_t v = cpx_div_smallq(xp, p);
+        if (v < qmax) qmax = v;
+      }
+      const uint64_t k1 = pi_lookup(bd.pitab, qmax);  // q indices (j, k1)
+      uint64_t s = 0;
+      for (uint64_t k = j + 1 + threadIdx.x; k < k1; k += blockDim.x) {
+        uint64_t u = cpx_div_smallq(xp, bd.primes[k]);
+        s += pi_lookup(bd.pitab, u);
+      }
+      sum += (i128)s;
+      if (threadIdx.x == 0 && k1 > j + 1)
+        sum += (i128)(k1 - j - 1) * (1 - (i128)j);     // (2 - pi(p)) per q, pi(p) = j + 1
+    }
+    i128 t = block_sum_i128(sum, red);
+    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = t;
+  }
+}
+
+hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_easy, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// -------------------------------------------------------- hard carry
+// For stage s (thread), walk the query's segments in order:
+//   carry += prefix * M[seg][s];  prefix += T[seg][s]
+// while the stage is active there (bact[seg] > s).
+__global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  const uint32_t nd = *bd.n_fin;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const BlockDesc d = bd.fin_desc[di];
+    const QueryDev &q = bd.queries[d.query];
+    const uint64_t s = d.begin + threadIdx.x;
+    i128 carry = 0;
+    if (s < d.end) {
+      uint64_t prefix = 0;
+      for (uint64_t r = 0; r < q.hard_nseg; r++) {
+        const uint64_t row = q.hard_row0 + r;
+        if (bd.hard_bact[row] <= s) break;
+        const uint64_t idx = q.tm_off + r * q.nstage + s;
+        carry += (i128)prefix * bd.hard_M[idx];
+        prefix += bd.hard_T[idx];
+      }
+    }
+    i128 t = block_sum_i128(carry, red);
+    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = t;
+  }
+}
+
+hipError_t launch_hard_finalize(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_hard_fin, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- combine
+// Block per query: P2 carry scan over P2 segments, hard local sums, then
+//   pi = S0 + (S1 + U) + V + S3 + pi(y) - 1 - P2.
+static __device__ __forceinline__ void put_i128(uint64_t out[2], i128 v)
+{
+  out[0] = (uint64_t)v;
+  out[1] = (uint64_t)((u128)v >> 64);
+}
+
+__global__ __launch_bounds__(256) void k_combine(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  __shared__ uint64_t wtot[4];
+  const uint32_t nq = *bd.nqueries;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const QueryDev &q = bd.queries[qi];
+    // P2: sum over segments of loc + np * (pi(p2_e) + primes in earlier segments)
+    i128 p2 = 0;
+    uint64_t base = q.pi_e;
+    for (uint64_t r0 = 0; r0 < q.p2_nseg; r0 += blockDim.x) {
+      uint64_t r = r0 + t;
+      uint64_t c = 0, np = 0, loc = 0;
+      if (r < q.p2_nseg) {
+        c = bd.p2_cnt[q.p2_row0 + r];
+        np = bd.p2_np[q.p2_row0 + r];
+        loc = bd.p2_loc[q.p2_row0 + r];
+      }
+      uint64_t incl = wave_incl_scan(c);
+      if (lane_id() == CPX_WAVE - 1) wtot[wave_id()] = incl;
+      __syncthreads();
+      uint64_t pre = incl - c, tot = 0;
+      for (uint32_t w = 0; w < blockDim.x / CPX_WAVE; w++) {
+        tot += wtot[w];
+        if (w < wave_id()) pre += wtot[w];
+      }
+      p2 += (i128)loc + (i128)np * (i128)(base + pre);
+      base += tot;
+      __syncthreads();
+    }
+    // hard: local sums
+    i128 hacc = 0;
+    for (uint64_t r = t; r < q.hard_nseg; r += blockDim.x) hacc += bd.hard_acc[q.hard_row0 + r];
+    i128 p2s = block_sum_i128(p2, red);
+    i128 hs = block_sum_i128(hacc, red);
+    if (t == 0 && q.small) {
+      ResultDev &res = bd.results[qi];
+      i128 pi = (i128)pi_lookup(bd.pitab, q.x_lo);
+      put_i128(res.pi, pi);
+      put_i128(res.ordinary, 0);
+      put_i128(res.trivial, 0);
+      put_i128(res.easy, 0);
+      put_i128(res.hard, 0);
+      put_i128(res.p2, 0);
+      res.pi_y = q.a;
+      res.status = 1;
+    } else if (t == 0) {
+      const i128 *pp = bd.parts + q.part_off;
+      i128 s0 = 0, u = 0, v = 0, hc = 0;
+      uint32_t k = 0;
+      for (uint32_t i = 0; i < q.n_ord_parts; i++) s0 += pp[k++];
+      for (uint32_t i = 0; i < q.n_triv_parts; i++) u += pp[k++];
+      for (uint32_t i = 0; i < q.n_easy_parts; i++) v += pp[k++];
+      for (uint32_t i = 0; i < q.n_hard_parts; i++) hc += pp[k++];
+      // S1 = C(pi(y) - pi(cbrt x), 2)
+      i128 kk = (i128)q.a - (i128)q.pi_cbrt;
+      i128 s1 = kk > 0 ? kk * (kk - 1) / 2 : 0;
+      i128 triv = s1 + u;
+      i128 hard = -(hs + hc);
+      // P2 = sum pi(x/p) - sum_{p} (pi(p) - 1), pi(p) = a+1 .. A
+      i128 A = (i128)q.pi_sqrtx, a = (i128)q.a;
+      i128 P2 = 0;
+      if (A > a) P2 = p2s - (A * (A - 1) / 2 - a * (a - 1) / 2);
+      i128 pi = s0 + triv + v + hard + a - 1 - P2;
+      ResultDev &res = bd.results[qi];
+      put_i128(res.pi, pi);
+      put_i128(res.ordinary, s0);
+      put_i128(res.trivial, triv);
+      put_i128(res.easy, v);
+      put_i128(res.hard, hard);
+      put_i128(res.p2, P2);
+      res.pi_y = q.a;
+      res.status = 1;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_combine(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_combine, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}

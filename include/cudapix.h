@@ -1,0 +1,225 @@
+/*
+ * cudapix.h -- C ABI of the MI355X combinatorial pi(x) engine
+ * (libcudapix_amd.so).
+ *
+ * The library replaces the pi(x) path of CUDApix: the entry points declared
+ * in include/P2.cuh, ordinary.cuh, trivial.cuh, phi.cuh, pitable.cuh, S3.cuh
+ * and V.cuh of the reference, plus the CUDASieve prime generation/counting and
+ * the LPF/Moebius sieves they call.  Each declaration below names the
+ * reference interface (file:line) it stands in for.  The reference API is
+ * overloaded C++ with no status codes and global device state; this ABI is
+ * plain C: every call takes a context, returns 0 (CPX_OK) or a negative
+ * CPX_E* code, never exits or resets the device, and is safe to use from
+ * different threads on different contexts.  128-bit values are passed as
+ * {lo, hi} pairs (signed ones in two's complement).
+ *
+ * Semantics are the mathematical values of each term of
+ *   pi(x) = S0 + (S1 + U) + V + S3 + pi(y) - 1 - P2      (src/Deleglise-Rivat/deleglise-rivat.cpp:77)
+ * with every bound taken by exact integer roots; where the reference's code
+ * is known to be off (its README.md:4,7) this library returns the exact value.
+ * C++ callers can include cudapix.hpp for overloads with the reference's
+ * names.
+ */
+#ifndef CUDAPIX_H
+#define CUDAPIX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t lo, hi; } cpx_u128;   /* unsigned 128-bit */
+typedef struct { uint64_t lo, hi; } cpx_i128;   /* signed 128-bit, two's complement */
+
+typedef struct cpx_ctx cpx_ctx;
+typedef struct cpx_pitable cpx_pitable;
+typedef struct cpx_phi cpx_phi;
+typedef struct cpx_server cpx_server;
+
+enum {
+  CPX_OK = 0,
+  CPX_EINVAL = -1,     /* bad argument (null pointer, malformed x, c out of 1..8) */
+  CPX_ENOMEM = -2,     /* device allocation failed or the HBM budget is exceeded */
+  CPX_EDEVICE = -3,    /* a HIP call failed; cpx_ctx_last_error() has the text */
+  CPX_ERANGE = -4,     /* x or y outside what the path supports */
+  CPX_EAGAIN = -5,     /* server: no result ready yet */
+  CPX_ECANCELED = -6,  /* server: request cancelled */
+  CPX_ENOTFOUND = -7,  /* server: unknown ticket */
+  CPX_ESTATE = -8      /* object used in a state that does not allow the call */
+};
+
+/* Context options.  Zero / negative fields take the defaults noted. */
+typedef struct {
+  int device;              /* HIP device ordinal (default 0) */
+  uint32_t c;              /* primes in the phi wheel, 1..8 (default 6) */
+  double alpha;            /* y = floor(alpha * cbrt(x)); <= 0: built-in alpha(x) model */
+  uint32_t seg_threads;    /* segment-kernel threads per block (0: tuned) */
+  uint32_t seg_kwords;     /* 32-bit words per thread per segment (0: tuned) */
+  uint64_t hbm_budget;     /* bytes the context may hold on the device (0: 90% of free) */
+  uint32_t fresh_tables;   /* 1: rebuild every table for each solve (one-at-a-time plain mode) */
+  uint32_t use_graphs;     /* 1 (default): replay batch launches from a captured hipGraph */
+} cpx_opts;
+
+/* Per-solve breakdown (all exact). */
+typedef struct {
+  cpx_u128 pi;
+  cpx_i128 ordinary;       /* S0 */
+  cpx_i128 trivial;        /* S1 + U */
+  cpx_i128 easy;           /* V */
+  cpx_i128 hard;           /* S3 */
+  cpx_i128 p2;             /* P2 */
+  uint64_t y, z, c, pi_y;
+  double alpha;            /* alpha actually used (y / cbrt(x)) */
+  double ms;               /* wall time of the solve */
+} cpx_terms;
+
+void cpx_opts_default(cpx_opts *o);
+int cpx_ctx_create(const cpx_opts *o, cpx_ctx **out);
+int cpx_ctx_destroy(cpx_ctx *ctx);
+const char *cpx_strerror(int code);
+const char *cpx_ctx_last_error(const cpx_ctx *ctx);
+/* y the engine would use for x under the context options. */
+int cpx_choose_y(cpx_ctx *ctx, cpx_u128 x, uint64_t *y);
+/* Device bytes currently held by the context (tables + batch buffers). */
+uint64_t cpx_ctx_device_bytes(const cpx_ctx *ctx);
+
+/* ---- full pi(x) -------------------------------------------------------- */
+/* deleglise_rivat64::pi_deleglise_rivat (src/Deleglise-Rivat/deleglise-rivat.cpp:50-80);
+ * y = 0 picks y from alpha. terms may be NULL. */
+int cpx_pi_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, uint32_t c, uint64_t *out, cpx_terms *terms);
+/* deleglise_rivat128::pi_deleglise_rivat (src/Deleglise-Rivat/deleglise-rivat.cpp:124-160) */
+int cpx_pi_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, uint32_t c, cpx_u128 *out, cpx_terms *terms);
+
+/* ---- terms (drop-in for the seven headers) ------------------------------ */
+/* uint64_t P2(uint64_t x, uint64_t y)            include/P2.cuh:16 */
+int cpx_p2_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, uint64_t *out);
+/* uint128_t P2(uint128_t x, uint64_t y)          include/P2.cuh:17 */
+int cpx_p2_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, cpx_u128 *out);
+/* void divXbyY(uint128_t x, uint64_t *y, size_t len)  include/P2.cuh:21 (device pointer, in place) */
+int cpx_div_x_by_y(cpx_ctx *ctx, cpx_u128 x, uint64_t *d_y, size_t len);
+/* uint64_t Ordinary(uint64_t x, uint64_t y, uint16_t c)    include/ordinary.cuh:6 */
+int cpx_ordinary_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, uint16_t c, int64_t *out);
+/* uint128_t Ordinary(uint128_t x, uint64_t y, uint16_t c)  include/ordinary.cuh:7 */
+int cpx_ordinary_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, uint16_t c, cpx_i128 *out);
+/* uint64_t S2_trivial(uint64_t x, uint64_t y)    include/trivial.cuh:8  (S1 + U) */
+int cpx_s2_trivial_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, uint64_t *out);
+/* uint128_t S2_trivial(uint128_t x, uint64_t y)  include/trivial.cuh:12 */
+int cpx_s2_trivial_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, cpx_u128 *out);
+/* uint64_t S3(uint64_t x, uint64_t y, uint32_t c)  include/S3.cuh:3 -- the DR hard
+ * leaves p_c < p <= x^(1/4) (the meaning deleglise_rivat64::S3 gives it,
+ * src/Deleglise-Rivat/S3.cu:6-9); signed result. */
+int cpx_s3_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, uint32_t c, int64_t *out);
+int cpx_s3_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, uint32_t c, cpx_i128 *out);
+/* int64_t V(uint64_t x, uint64_t y)               include/V.cuh:6 -- DR easy leaves */
+int cpx_v_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, int64_t *out);
+int cpx_v_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, cpx_i128 *out);
+/* int64_t V_a / V_b(x, pi, p, q)                  include/V.cuh:7-8: V split at p = sqrt(x/y) */
+int cpx_v_split_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, int64_t *v_a, int64_t *v_b);
+
+/* ---- primes, pi tables, phi (CUDASieve / pitable.cuh / phi.cuh) --------- */
+/* CudaSieve::countPrimes(bottom, top): primes in [lo, hi) */
+int cpx_count_primes(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t *out);
+/* CudaSieve::getDevicePrimes(bottom, top, len): device array of the primes in
+ * [lo, hi) (uint64), freed with cpx_free. */
+int cpx_get_device_primes(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len);
+/* uint32_t* get_d_piTable(uint32_t hi)   include/pitable.cuh:7: u32 pi(n) at
+ * index (n+1)/2 for n < hi; freed with cpx_free. */
+int cpx_get_d_pitable(cpx_ctx *ctx, uint32_t hi, uint32_t **d_out);
+/* class PiTable  include/pitable.cuh:12-51: sliding window [base, base+range)
+ * of u32 values pi(n) - pi_base at index (n + 1 - base) / 2 (base even). */
+int cpx_pitable_create(cpx_ctx *ctx, uint64_t base, uint64_t bottom, uint64_t range, cpx_pitable **out);
+int cpx_pitable_destroy(cpx_pitable *t);
+int cpx_pitable_current(cpx_pitable *t, uint32_t **d_table);   /* getCurrent */
+int cpx_pitable_next_up(cpx_pitable *t, uint32_t **d_table);   /* getNextUp */
+int cpx_pitable_next_down(cpx_pitable *t, uint32_t **d_table); /* getNextDown */
+int cpx_pitable_info(cpx_pitable *t, uint64_t *base, uint64_t *range, uint64_t *pi_base);
+/* class Phi  include/phi.cuh:17-36: phi(x, a) for x < max from an LPF table. */
+int cpx_phi_create(cpx_ctx *ctx, uint32_t max, cpx_phi **out);
+int cpx_phi_destroy(cpx_phi *p);
+int cpx_phi_eval(cpx_phi *p, uint32_t x, uint32_t a, uint64_t *out);          /* Phi::phi */
+/* Phi::generateRange(num, a): device u32 array, entry i = #{odd k <= 2i+1 : lpf(k) > p_a},
+ * freed with cpx_free. */
+int cpx_phi_range(cpx_phi *p, uint32_t num, uint32_t a, uint32_t **d_out);
+/* gen_d_lpf / gen_d_mu  include/sieve/lpf_mu.cuh:14-21: odd-only arrays for
+ * n = bottom + 2i + 1 < top (bottom even); lpf(1) = UINT32_MAX. */
+int cpx_gen_d_lpf(cpx_ctx *ctx, uint64_t bottom, uint64_t top, uint32_t **d_out);
+int cpx_gen_d_mu(cpx_ctx *ctx, uint64_t bottom, uint64_t top, int8_t **d_out);
+int cpx_free(cpx_ctx *ctx, void *d_ptr);
+
+/* ---- batched queries ----------------------------------------------------- */
+typedef struct {
+  uint32_t use_graph;      /* 1: replay a captured hipGraph of the batch launches */
+  uint32_t tune;           /* 1: let the autotuner pick the segment shape first */
+} cpx_batch_opts;
+
+typedef struct {
+  double ms_total;         /* wall time of the call */
+  double ms_plan;          /* host planning */
+  double ms_tables;        /* table (re)builds */
+  double ms_device;        /* device time of the batch (hipEvents) */
+  double ms_segments;      /* device time of the persistent segment kernel */
+  uint64_t tiles, hard_segments, p2_segments;
+  uint32_t seg_threads, seg_kwords;
+} cpx_batch_stats;
+
+/* Many queries in one launch sequence over shared resident tables: every
+ * hard-leaf and P2 segment of every query is a tile of one persistent kernel. */
+int cpx_pi_batch(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
+                 const cpx_batch_opts *opts, cpx_batch_stats *stats);
+/* Same, also returning each query's term breakdown (terms may be NULL). */
+int cpx_pi_batch_terms(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
+                       cpx_terms *terms, const cpx_batch_opts *opts, cpx_batch_stats *stats);
+
+/* ---- autotuner ----------------------------------------------------------- */
+/* Time the segment-kernel shapes on a calibration batch and keep the fastest
+ * for this context (also run by cpx_pi_batch when opts->tune is set). */
+int cpx_autotune(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, uint32_t *threads, uint32_t *kwords);
+
+/* ---- serving (continuous batching) --------------------------------------- */
+typedef struct {
+  uint64_t hbm_budget;     /* bytes of per-request sieve state admitted at once (0: ctx budget) */
+  uint32_t max_batch;      /* largest admission group (default 256) */
+  double max_wait_ms;      /* how long a lone request waits for company (default 0.5) */
+} cpx_server_opts;
+
+typedef struct {
+  uint64_t ticket;
+  cpx_u128 x;
+  cpx_u128 pi;
+  int status;              /* CPX_OK or a negative code */
+  double t_arrive_ms, t_admit_ms, t_done_ms;   /* server clock */
+} cpx_result;
+
+typedef struct {
+  uint64_t submitted, completed, rejected, cancelled, batches;
+  uint64_t max_inflight_bytes;
+  double p50_ms, p99_ms, mean_ms, qps;
+} cpx_server_stats;
+
+int cpx_server_create(cpx_ctx *ctx, const cpx_server_opts *o, cpx_server **out);
+int cpx_server_submit(cpx_server *s, cpx_u128 x, uint64_t *ticket);
+int cpx_server_cancel(cpx_server *s, uint64_t ticket);
+/* Up to max finished results; *n set to the number written. */
+int cpx_server_poll(cpx_server *s, cpx_result *out, size_t max, size_t *n);
+/* Block until every submitted request has finished. */
+int cpx_server_drain(cpx_server *s);
+int cpx_server_stats_get(cpx_server *s, cpx_server_stats *st);
+int cpx_server_destroy(cpx_server *s);
+/* Load generator: submit xs[i] at t_ms[i] milliseconds after the call (from
+ * the calling thread); returns once the last request is queued. */
+int cpx_server_replay(cpx_server *s, const cpx_u128 *xs, const double *t_ms, size_t n,
+                      uint64_t *tickets);
+/* Predicted device bytes of one request's sieve state (the admission price). */
+uint64_t cpx_request_bytes(cpx_ctx *ctx, cpx_u128 x);
+
+/* ---- host-side self checks (no GPU needed) -------------------------------- */
+/* Checks the library's 128/64 division, small-quotient division and integer
+ * roots against native __int128 on n pseudo-random cases; returns failures. */
+int cpx_selftest_arith(uint64_t seed, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUDAPIX_H */
