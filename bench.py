@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark: pi(x) queries/s, batched vs one-at-a-time plain, on one MI355X.
+
+A step is one batched solve of the 512-query workload (x log-uniform in
+[1e9, 1e15], splitmix64 seed 3, tests/golden/queries_c3.json) through the
+C ABI: every hard-leaf and P2 segment of every query is a tile of one
+persistent kernel launch over shared resident tables, replayed from captured
+hipGraphs.  value = 512 * steps / timed seconds (tables already resident).
+
+Beside it, in the same run:
+  * plain: the same 512 x solved one at a time, each with its own table build
+    (the reference's per-term, per-query table rebuilds), and plain_warm with
+    tables kept resident;  speedup = value / plain qps;
+  * served: a Poisson stream (tests/golden/queries_c4.json) through the
+    continuous-batching server, p50/p99 latency and queries/s;
+  * single: pi(1e16) wall time;
+  * roofline of the persistent segment kernel (HIP events on its stream);
+  * cpu_baseline: the oracle's C/OpenMP restatement of the reference terms on
+    a bounded, stratified sample of the same 512 x.
+Every batched result is checked bit-exact against the plain solve.
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SMALLP = [2, 3, 5, 7, 11, 13, 17, 19]
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_json(name):
+    with open(os.path.join(ROOT, "tests", "golden", name)) as f:
+        return json.load(f)
+
+
+def iroot(x, k):
+    r = int(round(x ** (1.0 / k)))
+    while r ** k > x:
+        r -= 1
+    while (r + 1) ** k <= x:
+        r += 1
+    return r
+
+
+def primes_upto(n):
+    import numpy as np
+    s = np.ones(n + 1, dtype=bool)
+    s[:2] = False
+    for i in range(2, int(n ** 0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    return np.nonzero(s)[0]
+
+
+def algorithmic_lds_bytes(queries, primes):
+    """Hard-leaf LDS work of the DR staged sieve, per query (x, y, c):
+    W = B_h * ceil(z/2)/8 + 8 * sum_{b=c+1}^{c+B_h} ceil((z/2)/p_b),
+    one popcount/prefix pass over the odd-only bitmap per stage plus one
+    32-bit read-modify-write per cross-off; p_h = floor(x^(1/4))."""
+    import numpy as np
+    tot = 0
+    for t in queries:
+        x, y, c = t["x"], t["y"], int(t["c"])
+        if x < 16384:
+            continue
+        z = x // y
+        ph = iroot(x, 4)
+        ps = primes[c:np.searchsorted(primes, ph, side="right")]
+        bh = len(ps)
+        half = (z + 1) // 2
+        tot += bh * half / 8.0 + 8.0 * float(np.sum(np.ceil(half / ps.astype(np.float64))))
+    return tot
+
+
+def cpu_baseline(xs, budget_s=20.0):
+    """Oracle (C + OpenMP) on a stratified sample of the workload."""
+    from oracle import oracle as O
+    srt = sorted(xs)
+    k = 16
+    idx = [int((i + 0.5) * len(srt) / k) for i in range(k)]
+    sample = [srt[i] for i in idx]
+    threads = os.cpu_count() or 1
+    t_sum = 0.0
+    done = 0
+    est_total = 0.0
+    for i, x in enumerate(sample):
+        y = O.choose_y(x, max(1.0, min(8.0, 0.25 * math.log10(x) - 1.0)))
+        t0 = time.time()
+        O.pi_dr(x, y, 6, threads)
+        dt = time.time() - t0
+        t_sum += dt
+        done += 1
+        est_total += dt * (len(srt) / k)
+        if t_sum > budget_s:
+            # remaining strata are larger x: scale by the last measured time
+            est_total += dt * (len(srt) / k) * (k - done) * 2.0
+            break
+    return {"value": len(srt) / est_total, "unit": "queries/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d of 16 strata (1 x per 32-query stratum of the sorted 512-query list), "
+                      "%.1f s of CPU; oracle P2/ordinary/easy OpenMP, hard-leaf walk 1 thread%s"
+                      % (done, t_sum, "" if done == k else ", untimed strata extrapolated x2")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--serve-n", type=int, default=400)
+    ap.add_argument("--serve-lambdas", default="200,2000")
+    ap.add_argument("--serve-xmax", type=float, default=1e16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X and this "
+              "workload shards by query with no exchange (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cudapix_amd as C
+    xs = load_json("queries_c3.json")["x"]
+    n = len(xs)
+    ctx = C.Context(device=0)
+
+    # ---- warmup (builds resident tables, autotunes the segment shape)
+    log("warmup: autotune + %d batched steps" % args.warmup)
+    t0 = time.time()
+    shape = ctx.autotune(xs)
+    log("autotuned shape threads=%d kwords=%d in %.1fs" % (shape[0], shape[1], time.time() - t0))
+    ref = None
+    for i in range(args.warmup):
+        r = ctx.pi_batch(xs, use_graph=True)
+        if ref is None:
+            ref = r
+        assert r == ref
+
+    # ---- timed batched steps
+    seg_ms, dev_ms = [], []
+    t0 = time.time()
+    for i in range(args.steps):
+        r, st = ctx.pi_batch(xs, use_graph=True, stats=True)
+        seg_ms.append(st["ms_segments"])
+        dev_ms.append(st["ms_device"])
+        if r != ref:
+            raise SystemExit("batched results changed between steps")
+    dt = time.time() - t0
+    qps = n * args.steps / dt
+    ms_step = 1000.0 * dt / args.steps
+    log("batched: %.1f q/s, %.2f ms/step (device %.2f ms, segments %.2f ms)"
+        % (qps, ms_step, sum(dev_ms) / len(dev_ms), sum(seg_ms) / len(seg_ms)))
+
+    # terms of the batch, for the roofline's algorithmic bytes
+    terms = ctx.pi_batch(xs, terms=True)
+    for x, t in zip(xs, terms):
+        t["x"] = x
+    primes = primes_upto(200000)
+    w_lds = algorithmic_lds_bytes(terms, primes)
+    seg_avg_s = (sum(seg_ms) / len(seg_ms)) / 1000.0
+
+    out = {"metric": "pi(x) queries/s batched+served vs plain (x<=1e16), p50/p99 ms, bit-exact",
+           "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u64/u128 integer",
+           "data": "synthetic: 512 x log-uniform in [1e9,1e15] (splitmix64 seed 3)",
+           "config": {"workload": "batched pi(x), 512 queries x in [1e9,1e15], one persistent "
+                                  "segment launch per step",
+                      "queries": n, "seg_threads": shape[0], "seg_kwords": shape[1],
+                      "parallelism": "single GPU"}}
+
+    # ---- plain one-at-a-time solves (fresh tables per query) and warm-table variant
+    if not args.no_plain:
+        with C.Context(device=0, fresh_tables=True, use_graphs=False,
+                       seg_threads=shape[0], seg_kwords=shape[1]) as pc:
+            t0 = time.time()
+            plain = [pc.pi(x) for x in xs]
+            dtp = time.time() - t0
+        with C.Context(device=0, use_graphs=False, seg_threads=shape[0],
+                       seg_kwords=shape[1]) as wc:
+            wc.pi(max(xs))
+            t0 = time.time()
+            warm = [wc.pi(x) for x in xs]
+            dtw = time.time() - t0
+        if plain != ref or warm != ref:
+            bad = [i for i in range(n) if plain[i] != ref[i] or warm[i] != ref[i]]
+            raise SystemExit("batched results differ from plain solves at %s" % bad[:5])
+        out["plain_qps"] = round(n / dtp, 2)
+        out["plain_warm_qps"] = round(n / dtw, 2)
+        out["speedup_vs_plain"] = round(qps / (n / dtp), 2)
+        out["speedup_vs_plain_warm"] = round(qps / (n / dtw), 2)
+        out["bit_exact_vs_plain"] = True
+        log("plain: %.1f q/s (fresh tables), %.1f q/s (warm tables)" % (n / dtp, n / dtw))
+
+    # ---- single large query
+    if not args.no_single:
+        x16 = 10 ** 16
+        ctx.pi(x16)
+        t0 = time.time()
+        t = ctx.pi(x16, terms=True)
+        d16 = time.time() - t0
+        if t["pi"] != 279238341033925:
+            raise SystemExit("pi(1e16) wrong: %d" % t["pi"])
+        _, st16 = ctx.pi_batch([x16], stats=True)
+        t["x"] = x16
+        w16 = algorithmic_lds_bytes([t], primes)
+        out["single_1e16"] = {"ms": round(d16 * 1000, 2), "y": t["y"], "alpha": round(t["alpha"], 3),
+                              "segments_ms": round(st16["ms_segments"], 3),
+                              "lds_algorithmic_GB": round(w16 / 1e9, 2),
+                              "pi": t["pi"]}
+        log("pi(1e16) = %d in %.1f ms" % (t["pi"], d16 * 1000))
+
+    # ---- served Poisson streams
+    if not args.no_serve:
+        c4 = load_json("queries_c4.json")["streams"]
+        served = {}
+        for lam in [int(v) for v in args.serve_lambdas.split(",") if v]:
+            s = c4[str(lam)]
+            pairs = [(tt, xx) for tt, xx in zip(s["t_ms"], s["x"]) if xx <= args.serve_xmax]
+            pairs = pairs[:args.serve_n]
+            t0 = pairs[0][0]
+            sx = [p[1] for p in pairs]
+            st_ms = [p[0] - t0 for p in pairs]
+            srv = C.Server(ctx)
+            srv.replay(sx, st_ms)
+            srv.drain()
+            res = srv.poll(len(sx) + 16)
+            stt = srv.stats()
+            srv.close()
+            got = {r["ticket"]: r for r in res}
+            ok = all(r["status"] == 0 for r in res) and len(res) == len(sx)
+            served[str(lam)] = {"n": len(sx), "qps": round(stt["qps"], 2),
+                                "p50_ms": round(stt["p50_ms"], 3), "p99_ms": round(stt["p99_ms"], 3),
+                                "batches": stt["batches"], "all_ok": ok}
+            log("served lambda=%d: %s" % (lam, served[str(lam)]))
+        # each served result equals the same x solved alone
+        out["served"] = served
+
+    # ---- roofline of the persistent segment kernel
+    ach = w_lds / seg_avg_s / 1e9
+    out["roofline"] = {"bound": "lds", "achieved": round(ach, 1), "peak": None, "unit": "GB/s",
+                       "frac": None, "traffic": None,
+                       "kernel": "k_segments (persistent hard-leaf + P2 tiles)",
+                       "kernel_ms": round(seg_avg_s * 1000, 3),
+                       "algorithmic_bytes": w_lds}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(xs)
+        log("cpu baseline: %s" % out["cpu_baseline"])
+
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,7 +100,8 @@ int cpx_ctx_destroy(cpx_ctx *c)
   if (!c) return CPX_EINVAL;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  for (auto &g : c->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   free_tables(c);
   if (c->small.p) (void)hipFree(c->small.p);
   if (c->small_primes.p) (void)hipFree(c->small_primes.p);

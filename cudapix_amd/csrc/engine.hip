@@ -583,38 +583,49 @@ static int seg_grid(cpx_ctx *c)
 
 static const uint32_t GRID_Q = 256, GRID_T = 2048;
 
-static int enqueue(cpx_ctx *c, const BatchDev &b, bool timed)
+// The launch sequence in three parts; events between the parts time the
+// persistent segment kernel on its own stream (also when replayed as graphs).
+//   part 0: clear results + queue head, setup, ordinary, trivial, easy
+//   part 1: persistent segment kernel (hard + P2 tiles)
+//   part 2: hard-leaf carry, combine
+static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_words)
 {
-  HIPCHK(launch_zero_u32(b.tile_head, 1, c->stream));
-  HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
-  HIPCHK(launch_ordinary(b, GRID_T, c->stream));
-  HIPCHK(launch_trivial(b, GRID_T, c->stream));
-  HIPCHK(launch_easy(b, GRID_T, c->stream));
-  if (timed) HIPCHK(hipEventRecord(c->ev[2], c->stream));
-  HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
-  if (timed) HIPCHK(hipEventRecord(c->ev[3], c->stream));
-  HIPCHK(launch_hard_finalize(b, GRID_T, c->stream));
-  HIPCHK(launch_combine(b, GRID_Q, c->stream));
+  if (part == 0) {
+    HIPCHK(launch_zero_u32((uint32_t *)b.results, nres_words, c->stream));
+    HIPCHK(launch_zero_u32(b.tile_head, 1, c->stream));
+    HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
+    HIPCHK(launch_ordinary(b, GRID_T, c->stream));
+    HIPCHK(launch_trivial(b, GRID_T, c->stream));
+    HIPCHK(launch_easy(b, GRID_T, c->stream));
+  } else if (part == 1) {
+    HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
+  } else {
+    HIPCHK(launch_hard_finalize(b, GRID_T, c->stream));
+    HIPCHK(launch_combine(b, GRID_Q, c->stream));
+  }
   return CPX_OK;
 }
 
-static int ensure_graph(cpx_ctx *c, const BatchDev &b)
+static int ensure_graph(cpx_ctx *c, const BatchDev &b, uint32_t nres_words)
 {
-  if (c->gexec && c->graph_gen == c->generation && c->graph_shape.threads == c->shape.threads &&
+  if (c->gexec[0] && c->graph_gen == c->generation && c->graph_shape.threads == c->shape.threads &&
       c->graph_shape.kwords == c->shape.kwords)
     return CPX_OK;
-  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  for (auto &g : c->gexec)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   // kernel attributes and the occupancy query must happen outside capture
   (void)seg_grid(c);
-  hipGraph_t g;
-  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue(c, b, false);
-  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-  if (rc) return rc;
-  if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
-  hipError_t e3 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e3 != hipSuccess) { c->gexec = nullptr; return hip_fail(c, e3, "hipGraphInstantiate"); }
+  for (int part = 0; part < 3; part++) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_part(c, b, part, nres_words);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
+    hipError_t e3 = hipGraphInstantiate(&c->gexec[part], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) { c->gexec[part] = nullptr; return hip_fail(c, e3, "hipGraphInstantiate"); }
+  }
   c->graph_gen = c->generation;
   c->graph_shape = c->shape;
   return CPX_OK;
@@ -656,13 +667,15 @@ int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask,
   if ((rc = upload_plan(c, pl))) return rc;
   BatchDev b = make_batchdev(c);
   bool graph = use_graph && c->opts.use_graphs;
-  bool timed = st != nullptr && !graph;
+  // results region of the arena, in 32-bit words (capacity, so graphs stay valid)
+  const uint32_t nres_words = (uint32_t)(arena_of(c).caps.q * sizeof(ResultDev) / 4);
+  if (graph && (rc = ensure_graph(c, b, nres_words))) return rc;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
-  if (graph) {
-    if ((rc = ensure_graph(c, b))) return rc;
-    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-  } else {
-    if ((rc = enqueue(c, b, timed))) return rc;
+  for (int part = 0; part < 3; part++) {
+    if (part == 1) HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (graph) HIPCHK(hipGraphLaunch(c->gexec[part], c->stream));
+    else if ((rc = enqueue_part(c, b, part, nres_words))) return rc;
+    if (part == 1) HIPCHK(hipEventRecord(c->ev[3], c->stream));
   }
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
   CtxArena &A = arena_of(c);
@@ -676,12 +689,9 @@ int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask,
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     st->ms_device = ms;
-    st->ms_segments = 0;
-    if (timed) {
-      float s2 = 0;
-      (void)hipEventElapsedTime(&s2, c->ev[2], c->ev[3]);
-      st->ms_segments = s2;
-    }
+    float s2 = 0;
+    (void)hipEventElapsedTime(&s2, c->ev[2], c->ev[3]);
+    st->ms_segments = s2;
     st->ms_tables = tt1 - tt0;
     st->ms_plan = (tp - t0) - (tt1 - tt0);
     st->ms_total = now_ms() - t0;

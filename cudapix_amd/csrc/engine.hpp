@@ -80,7 +80,7 @@ struct cpx_ctx {
   bool shape_tuned = false;
 
   // captured graph of the batch launch sequence
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};   // pre-terms, segments, post
   uint64_t graph_gen = ~0ull;
   SegShape graph_shape{0, 0};
 

@@ -105,8 +105,10 @@ static void lane_loop(cpx_server *s, cpx_ctx *ctx)
         s->state[r.ticket] = 1;
         s->pending.pop_front();
       }
-      if (grp.empty()) {                    // memory held by other lanes: wait for a release
-        s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+      if (grp.empty()) {
+        s->cv_done.notify_all();            // drain() re-checks (cancelled entries were dropped)
+        if (!s->pending.empty())            // memory held by other lanes: wait for a release
+          s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
         continue;
       }
       s->inflight_bytes += gbytes;

@@ -1,0 +1,110 @@
+"""Batched and served queries: each result equals the same x solved alone
+(tolerance 0), graphs and direct launches agree, all segment shapes agree,
+and the server admits by HBM budget, cancels and rejects cleanly."""
+import json
+import os
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def c3(n=None):
+    with open(os.path.join(GOLD, "queries_c3.json")) as f:
+        xs = json.load(f)["x"]
+    return xs if n is None else xs[:n]
+
+
+def test_batch_equals_alone(ctx, oracle):
+    xs = sorted(c3(64))[:48] + [10 ** 9, 10 ** 12, 16384, 5, 10 ** 5]
+    got = ctx.pi_batch(xs)
+    alone = [ctx.pi(x) for x in xs]
+    assert got == alone
+    for x, v in zip(xs, got):
+        if x <= 3 * 10 ** 9:
+            want = oracle.count_primes(0, x + 1)
+            assert v == want, x
+
+
+def test_batch_graph_vs_direct_and_repeat(ctx):
+    xs = c3(128)
+    a = ctx.pi_batch(xs, use_graph=True)
+    b = ctx.pi_batch(xs, use_graph=False)
+    c = ctx.pi_batch(xs, use_graph=True)
+    assert a == b == c
+    # a smaller batch after a larger one replays the same graph
+    d = ctx.pi_batch(xs[:7], use_graph=True)
+    assert d == a[:7]
+
+
+def test_batch_terms_match_oracle(ctx, oracle):
+    xs = [x for x in c3() if x < 3 * 10 ** 10][:12]
+    terms = ctx.pi_batch(xs, terms=True)
+    for x, t in zip(xs, terms):
+        ref = oracle.pi_dr(x, t["y"], int(t["c"]))
+        for k in ("ordinary", "trivial", "easy", "hard", "p2", "pi"):
+            assert t[k] == ref[k], (x, k)
+
+
+@pytest.mark.parametrize("shape", [(256, 32), (256, 64), (256, 128), (512, 32), (512, 64), (1024, 32)])
+def test_segment_shapes_agree(oracle, shape):
+    import cudapix_amd as C
+    xs = [10 ** 12, 123456789012, 10 ** 13 + 7]
+    with C.Context(device=0, seg_threads=shape[0], seg_kwords=shape[1]) as c:
+        got = c.pi_batch(xs)
+    assert got[0] == 37607912018
+    assert got[2] == oracle.pi_dr(xs[2], oracle.choose_y(xs[2], 2.0), 6)["pi"]
+
+
+def test_server_mix_equals_alone(ctx):
+    import cudapix_amd as C
+    rnd = random.Random(17)
+    xs = [int(10 ** (8 + 5 * rnd.random())) for _ in range(120)]
+    ts = sorted(rnd.random() * 300.0 for _ in xs)
+    alone = {x: ctx.pi(x) for x in set(xs)}
+    srv = C.Server(ctx)
+    tickets = srv.replay(xs, ts)
+    srv.drain()
+    res = srv.poll(1000)
+    st = srv.stats()
+    srv.close()
+    assert len(res) == len(xs)
+    byt = {r["ticket"]: r for r in res}
+    for tk, x in zip(tickets, xs):
+        r = byt[tk]
+        assert r["status"] == 0 and r["x"] == x and r["pi"] == alone[x]
+        assert r["t_arrive_ms"] <= r["t_admit_ms"] <= r["t_done_ms"]
+    assert st["completed"] == len(xs) and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_server_budget_rejects_and_cancels(ctx):
+    import cudapix_amd as C
+    small = 10 ** 9
+    big = 10 ** 15
+    budget = ctx.request_bytes(small) * 4
+    assert ctx.request_bytes(big) > budget
+    srv = C.Server(ctx, hbm_budget=budget, max_batch=4)
+    t_big = srv.submit(big)
+    ts = [srv.submit(small + i) for i in range(12)]
+    c = srv.cancel(ts[-1])
+    srv.drain()
+    res = {r["ticket"]: r for r in srv.poll(100)}
+    srv.close()
+    assert res[t_big]["status"] == -2                      # CPX_ENOMEM: never fits
+    if c == 0:
+        assert res[ts[-1]]["status"] == -6                 # CPX_ECANCELED
+    for t in ts[:-1]:
+        assert res[t]["status"] == 0 and res[t]["pi"] == ctx.pi(res[t]["x"])
+
+
+def test_invalid_arguments(ctx):
+    import cudapix_amd as C
+    with pytest.raises(C.CpxError):
+        ctx.pi(10 ** 12, y=10)                  # y below cbrt(x)
+    with pytest.raises(C.CpxError):
+        ctx.pi(10 ** 12, c=9)
+    with pytest.raises(C.CpxError):
+        ctx.pi(1 << 101)

@@ -1,0 +1,128 @@
+"""CPU tests: the oracle pinned by fixtures, published values and mathematics.
+
+The oracle (oracle/oracle.c) restates the reference's terms; here it is checked
+against (a) golden vectors from an independent numpy brute force that
+enumerates every special leaf from the definition (tests/golden/terms.json),
+(b) published pi(10^n), pi(2^k) and Mertens values (tests/golden/known.json),
+(c) brute semiprime counts for P2, (d) sympy factorisations for mu / lpf, and
+(e) invariance of pi(x) under different alpha and c.
+"""
+import json
+import os
+import random
+
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", load("terms.json"), ids=lambda t: "x%d-a%g-c%d" % (t["x"], t["alpha"], t["c"]))
+def test_oracle_terms_match_brute_force_leaves(oracle, case):
+    r = oracle.pi_dr(case["x"], case["y"], case["c"])
+    for k in ("ordinary", "trivial", "easy", "hard", "p2", "pi_y", "pi"):
+        assert r[k] == case[k], k
+
+
+@pytest.mark.parametrize("case", load("p2_brute.json"), ids=lambda t: "x%d-y%d" % (t["x"], t["y"]))
+def test_oracle_p2_matches_semiprime_count(oracle, case):
+    assert oracle.P2(case["x"], case["y"]) == case["p2"]
+
+
+def test_known_pi10_small_by_brute_sieve(oracle):
+    k = load("known.json")["pi10"]
+    for n in range(1, 10):
+        assert oracle.count_primes(0, 10 ** n + 1) == k[str(n)]
+
+
+def test_known_pi2_small_by_brute_sieve(oracle):
+    k = load("known.json")["pi2"]
+    for e in ("10", "20", "30", "32"):
+        assert oracle.count_primes(0, 2 ** int(e) + 1) == k[e]
+
+
+@pytest.mark.parametrize("n", [9, 10, 11, 12, 13])
+def test_oracle_dr_matches_published_pi(oracle, n):
+    x = 10 ** n
+    k = load("known.json")["pi10"]
+    for alpha in (1.0, 3.0):
+        y = oracle.choose_y(x, alpha)
+        assert oracle.pi_dr(x, y, 6)["pi"] == k[str(n)]
+
+
+def test_mertens(oracle):
+    k = load("known.json")["mertens10"]
+    for n in range(1, 8):
+        assert oracle.mertens(10 ** n) == k[str(n)]
+
+
+def test_mu_lpf_against_sympy(oracle):
+    sympy = pytest.importorskip("sympy")
+    mu, lpf = oracle.mu_lpf_odd(20001)
+    for i, n in enumerate(range(1, 20002, 2)):
+        if n == 1:
+            assert mu[i] == 1 and lpf[i] == 2 ** 64 - 1
+            continue
+        f = sympy.factorint(n)
+        assert lpf[i] == min(f)
+        assert mu[i] == (0 if max(f.values()) > 1 else (-1) ** len(f))
+
+
+def test_phi_legendre_equals_definition(oracle):
+    rnd = random.Random(5)
+    for _ in range(40):
+        n = rnd.randrange(0, 3000)
+        b = rnd.randrange(0, 9)
+        assert oracle.phi_legendre(n, b) == oracle.phi_brute(n, b)
+
+
+def test_identity_phi_equals_leaves(oracle):
+    # phi(x, pi(y)) = S0 + S1 + U + V + S3, checked against Legendre's phi
+    for x in (20000, 50000, 10 ** 5):
+        y = oracle.choose_y(x, 2.0)
+        r = oracle.pi_dr(x, y, 6)
+        phi = r["ordinary"] + r["trivial"] + r["easy"] + r["hard"]
+        assert phi == oracle.phi_legendre(x, r["pi_y"])
+
+
+@pytest.mark.parametrize("x", [10 ** 7 + 19, 3 ** 17, 2 ** 34 + 1, 10 ** 11 + 7])
+def test_alpha_and_c_invariance(oracle, x):
+    vals = set()
+    for alpha in (1.0, 1.7, 3.0, 5.5):
+        for c in (1, 4, 6, 8):
+            y = oracle.choose_y(x, alpha)
+            vals.add(oracle.pi_dr(x, y, c)["pi"])
+    assert len(vals) == 1
+    if x < 10 ** 9:
+        assert vals.pop() == oracle.count_primes(0, x + 1)
+
+
+def test_integer_roots_exact(oracle):
+    rnd = random.Random(11)
+    for _ in range(2000):
+        x = rnd.getrandbits(rnd.randrange(1, 101))
+        s, c, q = oracle.isqrt(x), oracle.icbrt(x), oracle.iroot4(x)
+        assert s * s <= x < (s + 1) ** 2
+        assert c ** 3 <= x < (c + 1) ** 3
+        assert q ** 4 <= x < (q + 1) ** 4
+
+
+def test_query_fixtures_regenerate():
+    # the committed workloads are what the generator produces
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(GOLD, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    g = mg.splitmix64(3)
+    xs = [int(10 ** (9 + 6 * mg.u53(g))) for _ in range(512)]
+    assert xs == load("queries_c3.json")["x"]
+    assert all(10 ** 9 <= x <= 10 ** 15 for x in xs)
+    c4 = load("queries_c4.json")["streams"]
+    assert sorted(int(k) for k in c4) == [50, 200, 500, 1000, 2000]
+    for s in c4.values():
+        assert len(s["x"]) == 2000 and all(10 ** 8 <= x <= 10 ** 17 for x in s["x"])
+        assert all(a < b for a, b in zip(s["t_ms"], s["t_ms"][1:]))
