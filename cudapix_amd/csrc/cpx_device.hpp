@@ -56,13 +56,20 @@ __device__ __forceinline__ i128 block_sum_i128(i128 v, i128 *scratch)
   return t;   // valid in thread 0
 }
 
-// Interleaved LDS layout of a segment bitmap: segment word g lives at
-// lds[(g % K) * T + g / K], so thread t owns the contiguous words
-// [t*K, (t+1)*K) and its j-th word sits at lds[j*T + t] (bank-conflict free
-// when every lane walks its own words in step).
+// Padded-linear LDS layout of a segment bitmap: segment word g lives at
+// lds[g + g / K].  Thread t owns the contiguous words [t*K, (t+1)*K); its
+// j-th word sits at lds[t*(K+1) + j], so lanes walking their own words in
+// step hit distinct banks (K+1 is odd), while block-wide strided cross-off
+// (lanes p bits apart) stays nearly linear.  Needs K*T + T words.
 __device__ __forceinline__ uint32_t seg_addr(uint32_t g, uint32_t K, uint32_t T)
 {
-  return (g % K) * T + g / K;
+  (void)T;
+  return g + g / K;
+}
+
+__device__ __forceinline__ uint32_t own_addr(uint32_t t, uint32_t j, uint32_t K)
+{
+  return t * (K + 1) + j;
 }
 
 // Bit offset, inside a segment whose bit i is the integer lo + 2i + 1 (lo even),

@@ -46,12 +46,14 @@ struct QueryDev {
   uint64_t sqrtx, cbrtx, qrtx, sqrtz;
   uint64_t p2_zt;         // floor(x / (y + 1)) : largest x/p over P2 primes
   uint64_t p2_e;          // even start of the P2 count range (sqrtx & ~1)
-  uint64_t tm_off;        // first element of this query in hard_T / hard_M
+  uint64_t tm_off;        // first element of this query in hard_T
   uint32_t c, B;          // hard stages b = c+1 .. B (1-based prime indices)
   uint32_t nstage;        // B - c
   uint32_t a;             // pi(y)
   uint32_t pi_cbrt, pi_qrt, pi_sqrtz, small;   // small: answered by one pi-table lookup
-  uint64_t hard_nseg, hard_row0;      // hard segments and first output row
+  uint64_t hard_nseg, hard_row0;      // hard segments and first output row (T, bact, rowinfo)
+  uint64_t m_off;                     // first element of this query in hard_M
+  uint64_t mrow_base, hard_nmrow;     // first M-row (acc slot) and M-row count (>= segments)
   uint64_t p2_nseg, p2_row0;          // P2 segments and first output row
   uint64_t stage_off;                 // first entry of this query in the stage table
   uint64_t part_off;                  // first slot in the partial-sum array
@@ -60,19 +62,49 @@ struct QueryDev {
   uint64_t pi_sqrtx;                  // written by setup: pi(sqrt x)
 };
 
-// One stage of the hard-leaf sieve: p_b and floor(x / p_b).
+// One stage of the hard-leaf sieve: p_b, floor(x / p_b) and constants that
+// turn the per-stage divisions into multiplications (computed once per query).
 struct StageDev {
-  uint64_t xp;
+  uint64_t xp;         // floor(x / p)
+  double inv_p;        // 1 / p rounded to f64
+  uint64_t magic;      // ceil(2^64 / p): a mod p for 32-bit a (Lemire, Kaser, Kurz 2019)
   uint32_t p;
-  uint32_t pad;
+  uint32_t ylo;        // floor(y / p)
 };
 
-// Work tile of the persistent segment kernel.
+// floor(a / d) from a precomputed inv ~ 1/d: exact when a < 2^53 or the
+// quotient is < 2^50 (one signed remainder correction).
+__host__ __device__ __forceinline__ uint64_t cpx_div_recip(uint64_t a, uint64_t d, double inv)
+{
+  uint64_t q = (uint64_t)((double)a * inv);
+  int64_t r = (int64_t)(a - q * d);
+  if (r < 0) q--;
+  else if ((uint64_t)r >= d) q++;
+  return q;
+}
+
+// a mod p for 32-bit a with magic = ceil(2^64 / p).
+__host__ __device__ __forceinline__ uint32_t cpx_fastmod32(uint32_t a, uint32_t p, uint64_t magic)
+{
+  uint64_t low = magic * a;
+  return (uint32_t)(((u128)low * p) >> 64);
+}
+
+// Work tile of the persistent segment kernel.  A leaf-heavy hard segment is
+// split into nrep replica tiles: each sieves the whole segment but evaluates
+// only the leaves with index = rep (mod nrep), writing its own sum(mu) row.
 enum { TILE_HARD = 0, TILE_P2 = 1 };
 struct TileDev {
   uint32_t query;
-  uint32_t kind;
+  uint16_t kind;
+  uint16_t rep;
   uint64_t seg;
+};
+
+// Per hard segment: its first M-row (query-relative) and number of replicas.
+struct HardRowInfo {
+  uint32_t mrow;
+  uint32_t nrep;
 };
 
 // Block descriptor of the grid-stride term kernels (ordinary, trivial, easy).
@@ -106,9 +138,12 @@ struct BatchDev {
   BlockDesc *triv_desc;  uint32_t *n_triv;
   BlockDesc *easy_desc;  uint32_t *n_easy;
   BlockDesc *fin_desc;   uint32_t *n_fin;     // hard carry blocks (begin = first stage)
+  HardRowInfo *rowinfo;                       // [row]
   // outputs
-  uint32_t *hard_T;      int32_t *hard_M;     // [row][stage]
-  uint32_t *hard_bact;   int64_t *hard_acc;   // [row]
+  uint32_t *hard_T;                           // [row][stage]  survivors before stage
+  int32_t *hard_M;                            // [mrow][stage] sum of mu of the leaves
+  uint32_t *hard_bact;                        // [row]   active stages
+  int64_t *hard_acc;                          // [mrow]  sum of mu * local phi
   uint32_t *p2_cnt;      uint32_t *p2_np;     uint64_t *p2_loc;   // [row]
   i128 *parts;           // partial sums
   ResultDev *results;

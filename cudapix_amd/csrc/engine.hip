@@ -366,26 +366,60 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
       pl.fin.push_back({(uint32_t)i, part++, b, std::min<uint64_t>(d.nstage, b + 256)});
     d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
     pl.parts += part;
-    // tiles with a rough relative cost (longest first)
+    // hard tiles: active stages per segment and, for the low (leaf-dense)
+    // segments, an estimate of the leaves to evaluate; a segment with many
+    // leaves becomes several replica tiles that split its leaves.
+    std::vector<uint64_t> xps;
+    if (d.hard_nseg) {
+      xps.resize(d.nstage);
+      for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (uint64_t)(h.x / c->h_primes[h.c + k]);
+    }
+    const double leaf_target = 256.0 * T;                // leaf evaluations per replica
+    uint32_t sparse_run = 0;
+    d.mrow_base = pl.hard_mrows;
+    d.m_off = pl.hard_mm;
+    uint64_t mrows = 0;
     for (uint64_t s = 0; s < d.hard_nseg; s++) {
-      uint64_t lo = s * 2 * SB;
-      uint64_t lim = h.qrtx;
+      const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
+      uint64_t xlo = ~0ull;
       if (lo) {
         u128 xl = h.x / lo;
-        uint64_t r = cpx_isqrt128(xl);
-        if (r < lim) lim = r;
+        xlo = (xl >> 64) ? ~0ull : (uint64_t)xl;
       }
-      uint64_t act = host_pi(c, lim);
-      act = act > h.c ? act - h.c : 0;
-      if (act > d.nstage) act = d.nstage;
-      double cost = 1.0 + (double)act * (1.0 + 8.0 * SB / (64.0 * 1024.0 * T));
-      tiles.push_back({cost, {(uint32_t)i, (uint32_t)TILE_HARD, s}});
+      // active stages: p_b^2 <= x / lo
+      uint64_t act = host_pi(c, std::min<uint64_t>(h.qrtx, lo ? cpx_isqrt128((u128)xlo) : h.qrtx));
+      act = act > h.c ? std::min<uint64_t>(act - h.c, d.nstage) : 0;
+      double leaves = 0, leafy = 0;
+      const bool est = sparse_run < 4;
+      for (uint32_t k = 0; est && k < act; k++) {
+        const uint64_t p = c->h_primes[h.c + k];
+        const uint64_t xp = xps[k];
+        uint64_t mhi = lo ? std::min<uint64_t>(h.y, xp / lo) : h.y;
+        uint64_t mlo = std::max<uint64_t>(xp / hi, h.y / p);
+        const bool sparse = p * p > h.y;
+        if (sparse) mlo = std::max<uint64_t>(mlo, p);
+        if (mhi > mlo) {
+          leafy += 1;
+          leaves += sparse ? (double)(mhi - mlo) / std::log((double)mhi) : 0.5 * (double)(mhi - mlo);
+        }
+      }
+      if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
+      uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
+      pl.rowinfo.push_back({(uint32_t)mrows, nrep});
+      mrows += nrep;
+      const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
+                          2.0 * leaves / ((double)nrep * T);
+      for (uint32_t r = 0; r < nrep; r++)
+        tiles.push_back({cost, {(uint32_t)i, (uint16_t)TILE_HARD, (uint16_t)r, s}});
     }
+    d.hard_nmrow = mrows;
+    pl.hard_mrows += mrows;
+    pl.hard_mm += mrows * d.nstage;
     for (uint64_t s = 0; s < d.p2_nseg; s++) {
       uint64_t hi = d.p2_e + (s + 1) * 2 * SB;
       double nsp = (double)host_pi(c, std::min<uint64_t>(cpx_isqrt128(hi), c->prime_limit));
       double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
-      tiles.push_back({cost, {(uint32_t)i, (uint32_t)TILE_P2, s}});
+      tiles.push_back({cost, {(uint32_t)i, (uint16_t)TILE_P2, (uint16_t)0, s}});
     }
     pl.hard_segments += d.hard_nseg;
     pl.p2_segments += d.p2_nseg;
@@ -402,11 +436,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
 // Fixed-capacity layout: array offsets depend only on capacities, so the
 // pointers baked into a captured graph stay valid until a capacity grows.
 struct Caps {
-  uint64_t q, stages, tiles, ord, triv, easy, fin, tm, rows, p2rows, parts;
+  uint64_t q, stages, tiles, ord, triv, easy, fin, tm, rows, p2rows, parts, mm, mrows;
 };
 
 struct Layout {
-  size_t counters, queries, stages, tiles, ord, triv, easy, fin, plan_bytes;
+  size_t counters, queries, stages, tiles, ord, triv, easy, fin, rowinfo, plan_bytes;
   size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
 };
 
@@ -424,12 +458,13 @@ static Layout make_layout(const Caps &k)
   L.triv = o;     o = al(o + k.triv * sizeof(BlockDesc));
   L.easy = o;     o = al(o + k.easy * sizeof(BlockDesc));
   L.fin = o;      o = al(o + k.fin * sizeof(BlockDesc));
+  L.rowinfo = o;  o = al(o + k.rows * sizeof(HardRowInfo));
   L.plan_bytes = o;
   o = 0;
   L.hard_T = o;    o = al(o + k.tm * 4);
-  L.hard_M = o;    o = al(o + k.tm * 4);
+  L.hard_M = o;    o = al(o + k.mm * 4);
   L.hard_bact = o; o = al(o + k.rows * 4);
-  L.hard_acc = o;  o = al(o + k.rows * 8);
+  L.hard_acc = o;  o = al(o + k.mrows * 8);
   L.p2_cnt = o;    o = al(o + k.p2rows * 4);
   L.p2_np = o;     o = al(o + k.p2rows * 4);
   L.p2_loc = o;    o = al(o + k.p2rows * 8);
@@ -446,6 +481,7 @@ static Caps needed_caps(const Plan &pl)
   k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size();
   k.ord = pl.ord.size(); k.triv = pl.triv.size(); k.easy = pl.easy.size(); k.fin = pl.fin.size();
   k.tm = pl.hard_tm; k.rows = pl.hard_rows; k.p2rows = pl.p2_rows; k.parts = pl.parts;
+  k.mm = pl.hard_mm; k.mrows = pl.hard_mrows;
   return k;
 }
 
@@ -479,6 +515,7 @@ static int ensure_arena(cpx_ctx *c, const Plan &pl)
   fit(k.q, std::max<uint64_t>(need.q, 64)); fit(k.stages, need.stages); fit(k.tiles, need.tiles);
   fit(k.ord, need.ord); fit(k.triv, need.triv); fit(k.easy, need.easy); fit(k.fin, need.fin);
   fit(k.tm, need.tm); fit(k.rows, need.rows); fit(k.p2rows, need.p2rows); fit(k.parts, need.parts);
+  fit(k.mm, need.mm); fit(k.mrows, need.mrows);
   if (!grow) return CPX_OK;
   Layout L = make_layout(k);
   if (c->d_plan) (void)hipFree(c->d_plan);
@@ -521,6 +558,7 @@ static BatchDev make_batchdev(cpx_ctx *c)
   b.triv_desc = (BlockDesc *)(P + L.triv);
   b.easy_desc = (BlockDesc *)(P + L.easy);
   b.fin_desc = (BlockDesc *)(P + L.fin);
+  b.rowinfo = (HardRowInfo *)(P + L.rowinfo);
   b.hard_T = (uint32_t *)(O + L.hard_T);
   b.hard_M = (int32_t *)(O + L.hard_M);
   b.hard_bact = (uint32_t *)(O + L.hard_bact);
@@ -552,6 +590,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   put(L.triv, pl.triv.data(), pl.triv.size() * sizeof(BlockDesc));
   put(L.easy, pl.easy.data(), pl.easy.size() * sizeof(BlockDesc));
   put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
+  put(L.rowinfo, pl.rowinfo.data(), pl.rowinfo.size() * sizeof(HardRowInfo));
   // copy only the used prefix of each array
   auto up = [&](size_t off, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
@@ -564,6 +603,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   HIPCHK(up(L.triv, pl.triv.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.easy, pl.easy.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.fin, pl.fin.size() * sizeof(BlockDesc)));
+  HIPCHK(up(L.rowinfo, pl.rowinfo.size() * sizeof(HardRowInfo)));
   return CPX_OK;
 }
 

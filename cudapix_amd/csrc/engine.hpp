@@ -36,7 +36,9 @@ struct Plan {
   std::vector<QueryDev> q;
   std::vector<TileDev> tiles;
   std::vector<BlockDesc> ord, triv, easy, fin;
+  std::vector<HardRowInfo> rowinfo;
   uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
+  uint64_t hard_mrows = 0, hard_mm = 0;
   uint64_t need_R = 0, need_P = 0, need_Y = 0;
   uint64_t hard_segments = 0, p2_segments = 0;
 };
@@ -76,7 +78,7 @@ struct cpx_ctx {
   uint64_t generation = 0;                      // bumps on any reallocation
 
   // segment-kernel shape
-  SegShape shape{256, 64};
+  SegShape shape{64, 64};
   bool shape_tuned = false;
 
   // captured graph of the batch launch sequence

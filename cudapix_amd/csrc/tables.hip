@@ -154,7 +154,7 @@ hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sp
   constexpr int T = 256, K = 32;                   // 2^18 odd bits per segment
   const uint64_t ent_per_seg = (uint64_t)K * T;
   uint64_t nseg = (nent + ent_per_seg - 1) / ent_per_seg;
-  size_t lds = (size_t)K * T * 4;
+  size_t lds = (size_t)(K + 1) * T * 4;
   hipLaunchKernelGGL((k_pitab_seg<T, K>), dim3((uint32_t)nseg), dim3(T), lds, s, tab, nent,
                      sprimes, nsprimes);
   hipError_t e = hipGetLastError();
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(T) void k_count_sieve(uint64_t lo, uint64_t hi,
   // count bits whose integer is an odd prime in [max(lo, 3), hi)
   unsigned long long c = 0;
   for (uint32_t j = 0; j < (uint32_t)K; j++) {
-    uint32_t w = lds[j * T + t];
+    uint32_t w = lds[own_addr(t, j, K)];
     uint64_t g = (uint64_t)t * K + j;
     uint64_t n0 = s0 + 64 * g + 1;                 // integer of bit 0 of this word
     if (n0 + 62 < lo || n0 >= hi) continue;
@@ -429,7 +429,7 @@ hipError_t launch_count_primes_sieve(uint64_t lo, uint64_t hi, const uint32_t *s
   uint64_t base = lo & ~1ull;
   uint64_t nseg = (hi - base + 2 * SB - 1) / (2 * SB);
   if (nseg > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_count_sieve<T, K>), dim3((uint32_t)nseg), dim3(T), (size_t)K * T * 4, s,
+  hipLaunchKernelGGL((k_count_sieve<T, K>), dim3((uint32_t)nseg), dim3(T), (size_t)(K + 1) * T * 4, s,
                      lo, hi, sprimes, nsp, out);
   return hipGetLastError();
 }

@@ -39,7 +39,9 @@ __global__ __launch_bounds__(256) void k_setup(BatchDev bd)
       uint32_t p = bd.primes[q.c + s];
       StageDev d;
       d.p = p;
-      d.pad = 0;
+      d.ylo = (uint32_t)(q.y / p);
+      d.inv_p = 1.0 / (double)p;
+      d.magic = ~0ull / p + 1;
       d.xp = cpx_div128_to64(x, p);
       bd.stages[q.stage_off + s] = d;
     }
@@ -182,9 +184,12 @@ __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
       for (uint64_t r = 0; r < q.hard_nseg; r++) {
         const uint64_t row = q.hard_row0 + r;
         if (bd.hard_bact[row] <= s) break;
-        const uint64_t idx = q.tm_off + r * q.nstage + s;
-        carry += (i128)prefix * bd.hard_M[idx];
-        prefix += bd.hard_T[idx];
+        const HardRowInfo ri = bd.rowinfo[row];
+        int64_t msum = 0;
+        for (uint32_t k = 0; k < ri.nrep; k++)
+          msum += bd.hard_M[q.m_off + ((uint64_t)ri.mrow + k) * q.nstage + s];
+        carry += (i128)prefix * msum;
+        prefix += bd.hard_T[q.tm_off + r * q.nstage + s];
       }
     }
     i128 t = block_sum_i128(carry, red);
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(256) void k_combine(BatchDev bd)
     }
     // hard: local sums
     i128 hacc = 0;
-    for (uint64_t r = t; r < q.hard_nseg; r += blockDim.x) hacc += bd.hard_acc[q.hard_row0 + r];
+    for (uint64_t r = t; r < q.hard_nmrow; r += blockDim.x) hacc += bd.hard_acc[q.mrow_base + r];
     i128 p2s = block_sum_i128(p2, red);
     i128 hs = block_sum_i128(hacc, red);
     if (t == 0 && q.small) {

@@ -1,0 +1,57 @@
+"""CPU test: a reference-style C++ caller compiles against include/cudapix.hpp
+(the drop-in for P2.cuh, ordinary.cuh, trivial.cuh, S3.cuh, V.cuh,
+pitable.cuh, phi.cuh), links libcudapix_amd.so and runs its host-only self
+check.  Device calls are compiled but not executed here."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CALLER = r"""
+#include "cudapix.hpp"
+#include <cstdio>
+// the reference's call sites, unchanged (src/Deleglise-Rivat/deleglise-rivat.cpp:57-67)
+uint64_t dr64(uint64_t x, uint64_t y, uint16_t c) {
+  uint64_t p2 = P2(x, y);
+  uint64_t ordinary = Ordinary(x, y, c);
+  uint64_t trivial = S2_trivial(x, y);
+  uint64_t easy = (uint64_t)V(x, y);
+  uint64_t hard = S3(x, y, c);
+  return ordinary + trivial + easy + hard - 1 - p2;   // + pi(y) by the caller
+}
+uint128_t dr128(uint128_t x, uint64_t y, uint16_t c) {
+  return Ordinary(x, y, c) + S2_trivial(x, y) - P2(x, y);
+}
+int main(int argc, char **argv) {
+  if (argc > 1) {                       // only with a device
+    PiTable t(1000000, 0, 1000000);
+    Phi phi(100000);
+    std::printf("%llu %llu\n", (unsigned long long)dr64(1000000000000ull, 20000, 6),
+                (unsigned long long)phi.phi(1000, 3));
+    (void)t.getCurrent();
+  }
+  std::printf("%d\n", cpx_selftest_arith(1, 10000));
+  return 0;
+}
+"""
+
+
+def test_reference_style_caller_builds_and_links(tmp_path):
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no g++")
+    lib = os.path.join(ROOT, "cudapix_amd", "libcudapix_amd.so")
+    assert os.path.exists(lib)
+    src = tmp_path / "caller.cpp"
+    src.write_text(CALLER)
+    exe = tmp_path / "caller"
+    cmd = [cxx, "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           "-L", os.path.join(ROOT, "cudapix_amd"), "-lcudapix_amd",
+           "-Wl,-rpath," + os.path.join(ROOT, "cudapix_amd"), "-L/opt/rocm/lib",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.strip().splitlines()[-1] == "0"

@@ -49,7 +49,8 @@ def test_batch_terms_match_oracle(ctx, oracle):
             assert t[k] == ref[k], (x, k)
 
 
-@pytest.mark.parametrize("shape", [(256, 32), (256, 64), (256, 128), (512, 32), (512, 64), (1024, 32)])
+@pytest.mark.parametrize("shape", [(64, 32), (64, 64), (64, 128), (128, 32), (128, 64), (256, 32),
+                                   (256, 64), (512, 32)])
 def test_segment_shapes_agree(oracle, shape):
     import cudapix_amd as C
     xs = [10 ** 12, 123456789012, 10 ** 13 + 7]
