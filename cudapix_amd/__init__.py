@@ -91,7 +91,7 @@ EXPORTS = [
     "cpx_pi_batch", "cpx_pi_batch_terms", "cpx_autotune",
     "cpx_server_create", "cpx_server_submit", "cpx_server_cancel", "cpx_server_poll",
     "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
-    "cpx_request_bytes", "cpx_selftest_arith",
+    "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -172,6 +172,7 @@ def lib():
                                              P(U64)]),
         "cpx_request_bytes": (U64, [vp, CpxU128]),
         "cpx_selftest_arith": (ctypes.c_int, [U64, U64]),
+        "cpx_selftest_wave": (ctypes.c_int, [vp, U64, ctypes.c_uint32, P(ctypes.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +385,12 @@ class Context:
 
     def device_bytes(self):
         return self._L.cpx_ctx_device_bytes(self.h)
+
+    def selftest_wave(self, seed=1, nwaves=4096):
+        """Wavefronts whose register scan/sum differ from a serial scan."""
+        bad = ctypes.c_uint32()
+        self._chk(self._L.cpx_selftest_wave(self.h, seed, nwaves, ctypes.byref(bad)), "selftest_wave")
+        return bad.value
 
 
 class PiTable:

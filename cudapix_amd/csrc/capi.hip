@@ -593,7 +593,7 @@ int cpx_get_device_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out
   if ((rc = cpx_count_primes(c, lo, hi, &cnt))) return rc;
   uint64_t *d = nullptr;
   HIPCHK(hipMalloc(&d, std::max<uint64_t>(cnt, 1) * 8));
-  hipError_t e = launch_extract_range_u64(c->pitab.p, lo, hi, d, c->stream);
+  hipError_t e = launch_extract_range_u64(c->pitab.p, lo, hi, d, std::max<uint64_t>(cnt, 1), c->stream);
   if (!e) e = hipStreamSynchronize(c->stream);
   if (e) { (void)hipFree(d); return hip_fail(c, e, "getDevicePrimes"); }
   *d_out = d;
@@ -728,3 +728,29 @@ int cpx_selftest_arith(uint64_t seed, uint64_t n)
 }
 
 }  // extern "C"
+
+int cpx_selftest_wave(cpx_ctx *c, uint64_t seed, uint32_t nwaves, uint32_t *bad)
+{
+  CTX_GUARD(c);
+  if (!bad || nwaves == 0 || nwaves > (1u << 20)) return CPX_EINVAL;
+  *bad = 0;
+  HIPCHK(hipSetDevice(c->device));
+  std::mt19937_64 g(seed);
+  std::vector<uint32_t> h((size_t)nwaves * 64);
+  for (auto &v : h) v = (uint32_t)g() >> (g() % 32);      // values up to 2^32 exercise the wrap
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, (h.size() + 1) * sizeof(uint32_t)));
+  int rc = CPX_OK;
+  do {
+    hipError_t e = hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = launch_zero_u32(d + h.size(), 1, c->stream);
+    if (!e) e = launch_selftest_wave(d, nwaves, d + h.size(), c->stream);
+    uint32_t nb = 0;
+    if (!e) e = hipMemcpyAsync(&nb, d + h.size(), 4, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) { rc = hip_fail(c, e, "selftest_wave"); break; }
+    *bad = nb;
+  } while (0);
+  (void)hipFree(d);
+  return rc;
+}

@@ -9,7 +9,31 @@
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (CPX_WAVE - 1); }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / CPX_WAVE; }
 
-// Inclusive prefix sum across the 64 lanes of a wavefront.
+// Inclusive prefix sum of 32-bit values across the 64 lanes of a wavefront in
+// registers (DPP): Hillis-Steele within each row of 16 lanes (row_shr 1, 2,
+// 4, 8, zero-filled at the row start), then row_bcast:15 adds row 0's total
+// to row 1 and row 2's to row 3, and row_bcast:31 adds lane 31's running
+// total to rows 2 and 3 -- the GCN cross-lane scan of AMD's GPUOpen note
+// "AMD GCN Assembly: Cross-Lane Operations" (2016).  The disabled rows of the
+// two broadcast steps keep their value.  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+  return v;
+}
+
+// Sum of 32-bit values over the wavefront, in every lane (DPP scan, then lane 63).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), CPX_WAVE - 1);
+}
+
+// Inclusive prefix sum across the 64 lanes of a wavefront (any width).
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v)
 {

@@ -101,10 +101,17 @@ struct TileDev {
   uint64_t seg;
 };
 
-// Per hard segment: its first M-row (query-relative) and number of replicas.
+// Per hard segment (query-relative): its first M-row, number of replicas,
+// number of active stages `act` (p^2 <= x/lo) and where its counters live:
+// T at hard_T[tm_off + toff + b], replica k's sum(mu) at
+// hard_M[m_off + moff + k*act + b], for stages b < act.
 struct HardRowInfo {
   uint32_t mrow;
   uint32_t nrep;
+  uint32_t act;
+  uint32_t pad;
+  uint64_t toff;
+  uint64_t moff;
 };
 
 // Block descriptor of the grid-stride term kernels (ordinary, trivial, easy).
@@ -137,11 +144,11 @@ struct BatchDev {
   BlockDesc *ord_desc;   uint32_t *n_ord;
   BlockDesc *triv_desc;  uint32_t *n_triv;
   BlockDesc *easy_desc;  uint32_t *n_easy;
-  BlockDesc *fin_desc;   uint32_t *n_fin;     // hard carry blocks (begin = first stage)
+  BlockDesc *fin_desc;   uint32_t *n_fin;     // hard carry blocks (begin = the stage)
   HardRowInfo *rowinfo;                       // [row]
   // outputs
-  uint32_t *hard_T;                           // [row][stage]  survivors before stage
-  int32_t *hard_M;                            // [mrow][stage] sum of mu of the leaves
+  uint32_t *hard_T;                           // per row, active stages: survivors before stage
+  int32_t *hard_M;                            // per mrow, active stages: sum of mu of the leaves
   uint32_t *hard_bact;                        // [row]   active stages
   int64_t *hard_acc;                          // [mrow]  sum of mu * local phi
   uint32_t *p2_cnt;      uint32_t *p2_np;     uint64_t *p2_loc;   // [row]
@@ -163,7 +170,7 @@ hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s);
 hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sprimes,
                                  uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
                                  hipStream_t s);
-hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes,
+hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes, uint64_t cap,
                                  hipStream_t s);
 hipError_t launch_mlpf(int32_t *mlpf, uint64_t nodd, const uint32_t *primes, uint32_t nsp,
                        hipStream_t s);
@@ -190,6 +197,7 @@ hipError_t launch_combine(BatchDev b, uint32_t maxq, hipStream_t s);
 
 // segsieve.hip
 hipError_t launch_segments(BatchDev b, SegShape shape, uint32_t grid, hipStream_t s);
+hipError_t launch_selftest_wave(const uint32_t *in, uint32_t n, uint32_t *bad, hipStream_t s);
 size_t seg_lds_bytes(SegShape shape);
 bool seg_shape_supported(SegShape shape);
 hipError_t launch_scan_entries(PiEntry *tab, uint64_t nent, uint32_t *tot, uint64_t *base,
@@ -204,4 +212,4 @@ hipError_t launch_mu_lpf_range(uint32_t *lpf, int8_t *mu, uint64_t ibase, uint64
 hipError_t launch_phi_prefix(const uint32_t *lpf, uint64_t n, uint32_t pa, uint32_t *out,
                              hipStream_t s);
 hipError_t launch_extract_range_u64(const PiEntry *tab, uint64_t lo, uint64_t hi, uint64_t *out,
-                                    hipStream_t s);
+                                    uint64_t cap, hipStream_t s);

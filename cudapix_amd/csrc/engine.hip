@@ -160,7 +160,7 @@ static int build_bootstrap(cpx_ctx *c)
   if ((rc = dbuf_reserve(c, c->small, 4096, &c->table_bytes))) return rc;
   if ((rc = dbuf_reserve(c, c->small_primes, 24000, &c->table_bytes))) return rc;
   HIPCHK(launch_small_sieve(c->small.p, 4096, c->stream));
-  HIPCHK(launch_extract_primes(c->small.p, (1u << 18) - 1, c->small_primes.p, c->stream));
+  HIPCHK(launch_extract_primes(c->small.p, (1u << 18) - 1, c->small_primes.p, 24000, c->stream));
   PiEntry last;
   HIPCHK(hipMemcpyAsync(&last, c->small.p + 4095, sizeof(last), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -245,7 +245,7 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
   uint32_t kk = (uint32_t)(((Pm & 63) + 1) >> 1);
   uint64_t np = 1 + e.count + (uint64_t)__builtin_popcount(e.bits & (uint32_t)((1ull << kk) - 1));
   if ((rc = dbuf_reserve(c, c->primes, np + 1, &c->table_bytes))) return rc;
-  HIPCHK(launch_extract_primes(c->pitab.p, Pm, c->primes.p, c->stream));
+  HIPCHK(launch_extract_primes(c->pitab.p, Pm, c->primes.p, np + 1, c->stream));
   c->h_primes.resize(np);
   HIPCHK(hipMemcpyAsync(c->h_primes.data(), c->primes.p, np * sizeof(uint32_t),
                         hipMemcpyDeviceToHost, c->stream));
@@ -322,7 +322,6 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     d.hard_row0 = pl.hard_rows;
     d.tm_off = pl.hard_tm;
     pl.hard_rows += d.hard_nseg;
-    pl.hard_tm += d.hard_nseg * d.nstage;
     // P2 segments over [p2_e, zt]
     d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? ceil_div(h.zt + 1 - d.p2_e, 2 * SB) : 0;
     d.p2_row0 = pl.p2_rows;
@@ -361,9 +360,8 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
       }
     }
     d.n_easy_parts = part - d.n_ord_parts - d.n_triv_parts;
-    // hard carry: one block per 256 stages
-    for (uint64_t b = 0; b < d.nstage; b += 256)
-      pl.fin.push_back({(uint32_t)i, part++, b, std::min<uint64_t>(d.nstage, b + 256)});
+    // hard carry: one block per stage
+    for (uint64_t b = 0; b < d.nstage; b++) pl.fin.push_back({(uint32_t)i, part++, b, b + 1});
     d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
     pl.parts += part;
     // hard tiles: active stages per segment and, for the low (leaf-dense)
@@ -378,7 +376,7 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     uint32_t sparse_run = 0;
     d.mrow_base = pl.hard_mrows;
     d.m_off = pl.hard_mm;
-    uint64_t mrows = 0;
+    uint64_t mrows = 0, toff = 0, moff = 0;
     for (uint64_t s = 0; s < d.hard_nseg; s++) {
       const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
       uint64_t xlo = ~0ull;
@@ -405,8 +403,10 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
       }
       if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
       uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
-      pl.rowinfo.push_back({(uint32_t)mrows, nrep});
+      pl.rowinfo.push_back({(uint32_t)mrows, nrep, (uint32_t)act, 0u, toff, moff});
       mrows += nrep;
+      toff += act;
+      moff += (uint64_t)nrep * act;
       const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
                           2.0 * leaves / ((double)nrep * T);
       for (uint32_t r = 0; r < nrep; r++)
@@ -414,7 +414,8 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     }
     d.hard_nmrow = mrows;
     pl.hard_mrows += mrows;
-    pl.hard_mm += mrows * d.nstage;
+    pl.hard_tm += toff;
+    pl.hard_mm += moff;
     for (uint64_t s = 0; s < d.p2_nseg; s++) {
       uint64_t hi = d.p2_e + (s + 1) * 2 * SB;
       double nsp = (double)host_pi(c, std::min<uint64_t>(cpx_isqrt128(hi), c->prime_limit));
@@ -680,7 +681,20 @@ uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &h)
   double B = q < 3 ? 1 : 1.26 * q / std::log(q);
   double hseg = std::ceil((double)(h.z + 1) / (2.0 * SB));
   double p2seg = std::ceil((double)(h.zt + 1) / (2.0 * SB));
-  double bytes = hseg * B * 8 + hseg * 12 + p2seg * 16 + 4096 + B * 16 + (h.y / 8192.0 + 16) * 48;
+  // active (segment, stage) pairs: segment r >= 1 (lo = 2 SB r) has about
+  // min(B, pi(sqrt(x / lo))) active stages; summed over log-spaced buckets
+  const double X = (double)h.x;
+  double pairs = B;
+  for (double r = 1; r < hseg;) {
+    const double r1 = std::min(hseg, std::max(r + 1, std::floor(r * 1.25)));
+    const double v = std::sqrt(X / (2.0 * SB * std::sqrt(r * r1)));
+    const double pv = v < 3 ? 0 : 1.26 * v / std::log(v);
+    pairs += (r1 - r) * std::min(B, pv);
+    r = r1;
+  }
+  // T and M per pair (M rows of replicated segments add a little), rowinfo,
+  // bact and the P2 rows, stage and carry descriptors, easy descriptors
+  double bytes = pairs * 8 * 1.1 + hseg * 48 + p2seg * 16 + 4096 + B * 80 + (h.y / 8192.0 + 16) * 48;
   return (uint64_t)bytes;
 }
 

@@ -47,9 +47,16 @@ struct SegShared {
   long long red_s[T / CPX_WAVE];
   union {
     uint32_t offs[P2_MAX_COOP];  // P2: first multiple of each block-shared prime
-    struct {
-      uint32_t dec[HARD_SCH];    // hard: survivors removed by each stage of the chunk
-      int32_t msum[HARD_SCH];    // hard: sum of mu of each stage's leaves (this replica)
+    struct {                     // hard: stage table of the current chunk
+      uint32_t dec[HARD_SCH];    // survivors removed by each stage
+      int32_t msum[HARD_SCH];    // sum of mu of each stage's leaves (this replica)
+      uint64_t magic[HARD_SCH];  // ceil(2^64 / p)
+      uint32_t p[HARD_SCH];
+      uint32_t oseg[HARD_SCH];   // first odd multiple of p in the segment (bit offset)
+      uint32_t cbase[HARD_SCH];  // leaf candidates: mtop/2 (dense) or first prime index
+      uint32_t ccnt[HARD_SCH];   // number of candidates | 2^31 if sparse (p^2 > y)
+      uint32_t lmask[HARD_SCH / 32];   // stages with candidates
+      uint32_t cend;             // first inactive stage of the chunk (or ~0)
     } st;
   } u;
   uint32_t tile;
@@ -96,23 +103,110 @@ __device__ __forceinline__ uint32_t read_wave_prefix(const SegShared<T, K> &sh, 
 }
 
 // Write the counters of stages [c0, c1) of the current chunk: T_b (replica 0
-// only), sum(mu) of this replica; then clear the chunk.  All threads call.
+// only), sum(mu) of this replica; then clear the chunk.  All threads call;
+// `running` is kept by wave 0.
 template <int T, int K>
 __device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uint32_t c1,
                                            uint32_t &running, uint32_t *outT, int32_t *outM,
                                            bool write_T)
 {
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (uint32_t k = c0; k < c1; k++) {
-      if (write_T) outT[k] = running;
-      running -= sh.u.st.dec[k - c0];
-      outM[k] = sh.u.st.msum[k - c0];
+  if (wave_id() == 0) {                 // wave 0 scans the removals 64 stages at a time
+    const uint32_t lane = lane_id();
+    for (uint32_t k0 = c0; k0 < c1; k0 += CPX_WAVE) {
+      const uint32_t k = k0 + lane;
+      const uint32_t d = k < c1 ? sh.u.st.dec[k - c0] : 0u;
+      const uint32_t incl = wave_incl_scan(d);
+      if (k < c1) {
+        if (write_T) outT[k] = running - (incl - d);
+        outM[k] = sh.u.st.msum[k - c0];
+      }
+      running -= __shfl(incl, CPX_WAVE - 1);
     }
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < HARD_SCH; k += T) { sh.u.st.dec[k] = 0; sh.u.st.msum[k] = 0; }
+  if (threadIdx.x < HARD_SCH / 32) sh.u.st.lmask[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.u.st.cend = ~0u;
   __syncthreads();
+}
+
+// Leaf candidate i of chunk stage k: dense stages read mu*lpf of m = mtop - 2i
+// (index mtop/2 - i), sparse stages read the i-th prime above mlo.
+__device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase, bool sparse,
+                                              uint32_t i)
+{
+  return sparse ? (int32_t)bd.primes[cbase + i] : bd.mlpf[cbase - i];
+}
+
+// Stage table of one chunk: for its stages k < clen (stage s0 + k) the
+// threads write p, the Lemire constant, the first odd multiple of p in the
+// segment (bit offset), the leaf candidate range and a leafy bit; the first
+// inactive stage (p^2 > x/lo) lowers st.cend.  Caller syncs before and after.
+template <int T, int K>
+__device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const QueryDev &q,
+                                                 const StageDev *st, uint32_t s0, uint32_t clen,
+                                                 uint64_t lo, uint64_t hi, uint64_t xlo,
+                                                 double inv_lo, double inv_hi,
+                                                 SegShared<T, K> &sh)
+{
+  const uint64_t y = q.y;
+  for (uint32_t k = threadIdx.x; k < clen; k += T) {
+    const StageDev sd = st[s0 + k];
+    const uint64_t p = sd.p, xp = sd.xp;
+    if (p * p > xlo) {                       // inactive: so are all later stages
+      atomicMin(&sh.u.st.cend, k);
+      continue;
+    }
+    sh.u.st.p[k] = (uint32_t)p;
+    sh.u.st.magic[k] = sd.magic;
+    // first odd multiple of p at or above lo + 1, as a bit offset (< p)
+    const uint64_t start = lo + 1;
+    const uint64_t rr = start - cpx_div_recip(start, p, sd.inv_p) * p;
+    uint64_t mm = start + (rr ? p - rr : 0);
+    if (!(mm & 1)) mm += p;
+    sh.u.st.oseg[k] = (uint32_t)((mm - start) >> 1);
+    // m range of this segment's leaves of p: x/(p hi) < m <= x/(p lo), m > y/p
+    const bool sparse = p * p > y;
+    uint64_t mhi = y;
+    if (lo != 0 && (xp >> 50) < lo) {
+      const uint64_t v = cpx_div_recip(xp, lo, inv_lo);
+      if (v < mhi) mhi = v;
+    }
+    uint64_t mlo = ~0ull;
+    if ((xp >> 50) < hi) mlo = cpx_div_recip(xp, hi, inv_hi);
+    uint32_t ncand = 0, cbase = 0;
+    if (mlo < mhi) {
+      if (mlo < sd.ylo) mlo = sd.ylo;
+      if (sparse && mlo < p) mlo = p;
+      if (mhi > mlo) {
+        if (!sparse) {
+          const uint64_t mtop = mhi - ((mhi & 1) ^ 1);            // largest odd <= mhi
+          ncand = mtop > mlo ? (uint32_t)((mtop - mlo + 1) / 2) : 0;
+          cbase = (uint32_t)(mtop >> 1);
+        } else {
+          const uint64_t il = pi_lookup(bd.pitab, mlo);
+          ncand = (uint32_t)(pi_lookup(bd.pitab, mhi) - il);
+          cbase = (uint32_t)il;
+        }
+      }
+    }
+    sh.u.st.cbase[k] = cbase;
+    sh.u.st.ccnt[k] = ncand | (sparse ? 0x80000000u : 0u);
+    if (ncand) atomicOr(&sh.u.st.lmask[k >> 5], 1u << (k & 31));
+  }
+}
+
+// First leafy stage of the chunk at or after k (HARD_SCH if none).
+template <int T, int K>
+__device__ __forceinline__ uint32_t hard_next_leafy(const SegShared<T, K> &sh, uint32_t k)
+{
+  for (uint32_t w = k >> 5; w < HARD_SCH / 32; w++) {
+    uint32_t m = sh.u.st.lmask[w];
+    if (w == (k >> 5)) m &= ~cpx_lowmask32(k & 31);
+    if (m) return w * 32 + (uint32_t)__builtin_ctz(m);
+  }
+  return HARD_SCH;
 }
 
 template <int T, int K>
@@ -127,31 +221,18 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   const uint64_t lo = seg * 2 * SB;
   const uint64_t hi = lo + 2 * SB;
   const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
-  const uint64_t y = q.y;
   const WheelInfo wi = bd.winfo[q.c];
   const uint64_t row = q.hard_row0 + seg;
   const HardRowInfo ri = bd.rowinfo[row];
   const uint32_t rep = tile.rep, nrep = ri.nrep;
   const uint64_t mrow = (uint64_t)ri.mrow + rep;
-  uint32_t *outT = bd.hard_T + q.tm_off + seg * q.nstage;
-  int32_t *outM = bd.hard_M + q.m_off + mrow * q.nstage;
+  // counters of the active stages only: the plan sized this row for ri.act
+  // stages, and the stage loop stops there at the latest
+  const uint32_t nst = min(q.nstage, ri.act);
+  uint32_t *outT = bd.hard_T + q.tm_off + ri.toff;
+  int32_t *outM = bd.hard_M + q.m_off + ri.moff + (uint64_t)rep * ri.act;
   const StageDev *st = bd.stages + q.stage_off;
-  const int32_t *__restrict__ mlpf = bd.mlpf;
-
-  seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
-  for (uint32_t k = t; k < HARD_SCH; k += T) { sh.u.st.dec[k] = 0; sh.u.st.msum[k] = 0; }
-  __syncthreads();
-  uint32_t cnt = 0;
-#pragma unroll 8
-  for (uint32_t j = 0; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
-  {
-    uint32_t ws = wave_sum(cnt);
-    if (lane == 0) sh.wtot[wv] = ws;
-  }
-  __syncthreads();
-  uint32_t running = 0;                                 // survivors before the next stage
-  for (uint32_t w = 0; w < NW; w++) running += sh.wtot[w];
-  __syncthreads();
+  const uint32_t i0 = rep * T + t, step = T * nrep;     // this thread's leaf candidates
 
   // stage b is active while p_b^2 <= x / lo (all leaves of p_b have n < x/p_b^2)
   uint64_t xlo = ~0ull;
@@ -161,92 +242,100 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   }
   const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
 
+  seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
+  for (uint32_t k = t; k < HARD_SCH; k += T) { sh.u.st.dec[k] = 0; sh.u.st.msum[k] = 0; }
+  if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
+  if (t == 0) sh.u.st.cend = ~0u;
+  __syncthreads();
+  uint32_t cnt = 0;
+#pragma unroll 8
+  for (uint32_t j = 0; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
+  {
+    uint32_t ws = wave_sum(cnt);
+    if (lane == 0) sh.wsum[wv] = (int32_t)ws;           // wtot is rewritten by the stages
+  }
+  uint32_t s0 = 0, clen = min((uint32_t)HARD_SCH, nst);
+  hard_chunk_table<T, K>(bd, q, st, s0, clen, lo, hi, xlo, inv_lo, inv_hi, sh);
+  __syncthreads();
+  uint32_t running = 0;                                 // survivors before the next stage
+  for (uint32_t w = 0; w < NW; w++) running += (uint32_t)sh.wsum[w];
+
   int64_t acc = 0;
-  uint32_t s = 0, chunk0 = 0;
-  StageDev cur = st[0];
-  for (; s < q.nstage; s++) {
-    const uint32_t p = cur.p;
-    const uint64_t xp = cur.xp;
-    if ((uint64_t)p * p > xlo) break;
-    const StageDev nxt = (s + 1 < q.nstage) ? st[s + 1] : cur;
-
-    // m range of this segment's leaves of p (uniform over the block)
-    const bool sparse = (uint64_t)p * p > y;
-    uint64_t mhi = y;
-    if (lo != 0 && (xp >> 50) < lo) {
-      uint64_t v = cpx_div_recip(xp, lo, inv_lo);
-      if (v < mhi) mhi = v;
+  uint32_t nact = 0;                                    // active stages of the segment
+  for (;;) {
+    const uint32_t cend = min(clen, sh.u.st.cend);
+    // one-ahead prefetch of this thread's first candidate of the next leafy stage
+    uint32_t pf_k = hard_next_leafy<T, K>(sh, 0);
+    int32_t pf = 0;
+    if (pf_k < cend) {
+      const uint32_t cc = sh.u.st.ccnt[pf_k];
+      if (i0 < (cc & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc >> 31, i0);
     }
-    uint64_t mlo = ~0ull;
-    if ((xp >> 50) < hi) mlo = cpx_div_recip(xp, hi, inv_hi);
-    if (mlo <= mhi) {
-      if (mlo < cur.ylo) mlo = cur.ylo;
-      if (sparse && mlo < p) mlo = p;
-    }
-
-    if (mhi > mlo) {
-      // ---- leaves of p (state: crossed by p_1 .. p_{b-1}); needs every
-      //      thread's words and counts, so the block meets here
-      publish_counts<T, K>(cnt, sh);
-      __syncthreads();
-      uint32_t wp[NW];
-      (void)read_wave_prefix<T, K>(sh, wp);
-      int32_t smu = 0;
-      const uint64_t step = (uint64_t)T * nrep;
-      if (!sparse) {
-        const uint64_t mtop = mhi - ((mhi & 1) ^ 1);            // largest odd <= mhi
-        const uint64_t ncand = mtop > mlo ? (mtop - mlo + 1) / 2 : 0;
-#pragma unroll 4
-        for (uint64_t i = (uint64_t)rep * T + t; i < ncand; i += step) {
-          const uint64_t m = mtop - 2 * i;
-          const int32_t v = mlpf[m >> 1];
-          const uint32_t av = (uint32_t)(v < 0 ? -v : v);
-          if (av <= p) continue;                               // mu(m) = 0 or lpf(m) <= p
-          const uint64_t n = cpx_div_smallq(xp, m);           // lo <= n < hi
+    uint32_t lw = 0;
+    for (uint32_t k = 0; k < cend; k++) {
+      if ((k & 31) == 0) lw = sh.u.st.lmask[k >> 5];
+      const uint32_t p = sh.u.st.p[k];
+      if ((lw >> (k & 31)) & 1u) {
+        // ---- leaves of p (state: crossed by p_1 .. p_{b-1}); needs every
+        //      thread's words and counts, so the block meets here
+        publish_counts<T, K>(cnt, sh);
+        __syncthreads();
+        uint32_t wp[NW];
+        (void)read_wave_prefix<T, K>(sh, wp);
+        const uint32_t cc = sh.u.st.ccnt[k], cb = sh.u.st.cbase[k];
+        const bool sparse = cc >> 31;
+        const uint32_t ncand = cc & 0x7fffffffu;
+        const uint64_t xp = st[s0 + k].xp;
+        int32_t smu = 0;
+        int32_t v = pf;
+        const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
+        for (uint32_t i = i0; i < ncand; i += step) {
+          if (i != i0) v = hard_fetch(bd, cb, sparse, i);
+          uint64_t m;
+          if (sparse) {
+            m = (uint32_t)v;                                  // prime, mu = -1
+            v = -1;
+          } else {
+            const uint32_t av = (uint32_t)(v < 0 ? -v : v);
+            if (av <= p) continue;                            // mu(m) = 0 or lpf(m) <= p
+            m = 2 * (uint64_t)(cb - i) + 1;
+          }
+          const uint64_t n = cpx_div_smallq(xp, m);         // lo <= n < hi
           const uint32_t c = count_below<T, K>(lds, sh, wp, (uint32_t)((n - lo + 1) >> 1));
           if (v > 0) { acc += c; smu++; } else { acc -= c; smu--; }
         }
-      } else {
-        const uint64_t ih = pi_lookup(bd.pitab, mhi), il = pi_lookup(bd.pitab, mlo);
-#pragma unroll 4
-        for (uint64_t k = il + (uint64_t)rep * T + t; k < ih; k += step) {
-          const uint64_t m = bd.primes[k];                     // prime, mu = -1
-          const uint64_t n = cpx_div_smallq(xp, m);
-          acc -= count_below<T, K>(lds, sh, wp, (uint32_t)((n - lo + 1) >> 1));
-          smu--;
+        pf_k = nk;
+        if (nk < cend) {
+          const uint32_t c2 = sh.u.st.ccnt[nk];
+          if (i0 < (c2 & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[nk], c2 >> 31, i0);
         }
+        if (smu) atomicAdd(&sh.u.st.msum[k], smu);
+        __syncthreads();                                      // readers done before cross-off
       }
-      if (smu) atomicAdd(&sh.u.st.msum[s - chunk0], smu);
-      __syncthreads();                                          // readers done before cross-off
-    }
-
-    // ---- cross off the odd multiples of p in this thread's own bits
-    {
-      // first odd multiple of p at or above lo + 1, as a bit offset (< p)
-      const uint64_t start = lo + 1;
-      const uint64_t rr = start - cpx_div_recip(start, p, cur.inv_p) * p;
-      uint64_t mm = start + (rr ? p - rr : 0);
-      if (!(mm & 1)) mm += p;
-      const uint32_t oseg = (uint32_t)((mm - start) >> 1);
-      const uint32_t tb = cpx_fastmod32(t * BPT, p, cur.magic);
-      uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
-      const uint32_t a0 = t * (K + 1);
-      uint32_t d = 0;
-      for (; j < BPT; j += p) {
-        const uint32_t addr = a0 + (j >> 5);
-        const uint32_t bit = 1u << (j & 31u);
-        const uint32_t w = lds[addr];
-        if (w & bit) { lds[addr] = w & ~bit; d++; }
+      // ---- cross off the odd multiples of p in this thread's own bits
+      {
+        const uint32_t oseg = sh.u.st.oseg[k];
+        const uint32_t tb = cpx_fastmod32(t * BPT, p, sh.u.st.magic[k]);
+        uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
+        const uint32_t a0 = t * (K + 1);
+        uint32_t d = 0;
+        for (; j < BPT; j += p) {
+          const uint32_t addr = a0 + (j >> 5);
+          const uint32_t bit = 1u << (j & 31u);
+          const uint32_t w = lds[addr];
+          if (w & bit) { lds[addr] = w & ~bit; d++; }
+        }
+        if (d) { atomicAdd(&sh.u.st.dec[k], d); cnt -= d; }
       }
-      if (d) { atomicAdd(&sh.u.st.dec[s - chunk0], d); cnt -= d; }
     }
-    if (s + 1 - chunk0 == HARD_SCH) {
-      hard_flush<T, K>(sh, chunk0, s + 1, running, outT, outM, rep == 0);
-      chunk0 = s + 1;
-    }
-    cur = nxt;
+    nact = s0 + cend;
+    hard_flush<T, K>(sh, s0, nact, running, outT, outM, rep == 0);
+    if (cend < clen || nact >= nst) break;
+    s0 = nact;
+    clen = min((uint32_t)HARD_SCH, nst - s0);
+    hard_chunk_table<T, K>(bd, q, st, s0, clen, lo, hi, xlo, inv_lo, inv_hi, sh);
+    __syncthreads();
   }
-  hard_flush<T, K>(sh, chunk0, s, running, outT, outM, rep == 0);
   long long a = wave_sum((long long)acc);
   if (lane == 0) sh.red_s[wv] = a;
   __syncthreads();
@@ -254,7 +343,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
     long long tt = 0;
     for (uint32_t w = 0; w < NW; w++) tt += sh.red_s[w];
     bd.hard_acc[q.mrow_base + mrow] = tt;
-    if (rep == 0) bd.hard_bact[row] = s;
+    if (rep == 0) bd.hard_bact[row] = nact;
   }
 }
 
@@ -419,4 +508,32 @@ hipError_t launch_segments(BatchDev b, SegShape shape, uint32_t grid, hipStream_
   CPX_SEG_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+// Self check of the register wavefront scan and sum against a serial scan
+// through LDS: one 64-lane wavefront per block, n blocks over in[64 * n].
+__global__ __launch_bounds__(CPX_WAVE) void k_selftest_wave(const uint32_t *__restrict__ in,
+                                                            uint32_t n, uint32_t *bad)
+{
+  __shared__ uint32_t v[CPX_WAVE];
+  const uint32_t l = threadIdx.x;
+  if (blockIdx.x >= n) return;
+  const uint32_t a = in[(uint64_t)blockIdx.x * CPX_WAVE + l];
+  v[l] = a;
+  __syncthreads();
+  uint32_t ref = 0, tot = 0;
+  for (uint32_t k = 0; k < CPX_WAVE; k++) {
+    if (k <= l) ref += v[k];
+    tot += v[k];
+  }
+  const uint32_t incl = wave_incl_scan(a);
+  const uint32_t sum = wave_sum(a);
+  if (incl != ref || sum != tot) atomicAdd(bad, 1u);
+}
+
+hipError_t launch_selftest_wave(const uint32_t *in, uint32_t n, uint32_t *bad, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_selftest_wave, dim3(n), dim3(CPX_WAVE), 0, s, in, n, bad);
+  return hipGetLastError();
 }

@@ -165,7 +165,7 @@ hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sp
 // ------------------------------------------------------------ prime list
 // primes[0] = 2, then every odd prime <= limit in order, at index count+1+rank.
 __global__ __launch_bounds__(256) void k_extract(const PiEntry *__restrict__ tab, uint64_t limit,
-                                                 uint32_t *primes)
+                                                 uint32_t *primes, uint64_t cap)
 {
   uint64_t nent = limit / 64 + 1;
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nent;
@@ -177,20 +177,21 @@ __global__ __launch_bounds__(256) void k_extract(const PiEntry *__restrict__ tab
       uint32_t i = __builtin_ctz(b);
       uint64_t n = 64 * w + 2 * i + 1;
       if (n > limit) break;
-      primes[k++] = (uint32_t)n;
+      if (k < cap) primes[k] = (uint32_t)n;
+      k++;
       b &= b - 1;
     }
-    if (w == 0) primes[0] = 2;
+    if (w == 0 && cap) primes[0] = 2;
   }
 }
 
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes,
-                                 hipStream_t s)
+                                 uint64_t cap, hipStream_t s)
 {
   uint64_t nent = limit / 64 + 1;
   uint32_t g = (uint32_t)((nent + 255) / 256);
   if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(k_extract, dim3(g), dim3(256), 0, s, tab, limit, primes);
+  hipLaunchKernelGGL(k_extract, dim3(g), dim3(256), 0, s, tab, limit, primes, cap);
   return hipGetLastError();
 }
 
@@ -509,7 +510,7 @@ hipError_t launch_phi_prefix(const uint32_t *lpf, uint64_t n, uint32_t pa, uint3
 
 // ------------------------------------------ primes of [lo, hi) as uint64
 __global__ __launch_bounds__(256) void k_extract_range(const PiEntry *__restrict__ tab, uint64_t lo,
-                                                       uint64_t hi, uint64_t *out)
+                                                       uint64_t hi, uint64_t *out, uint64_t cap)
 {
   const uint64_t base = lo >= 1 ? pi_lookup(tab, lo - 1) : 0;     // primes < lo
   const uint64_t w0 = lo >> 6, w1 = (hi - 1) >> 6;
@@ -522,20 +523,20 @@ __global__ __launch_bounds__(256) void k_extract_range(const PiEntry *__restrict
       uint32_t i = __builtin_ctz(b);
       uint64_t n = 64 * w + 2 * i + 1;
       b &= b - 1;
-      if (n >= lo && n < hi) out[k - base] = n;
+      if (n >= lo && n < hi && k - base < cap) out[k - base] = n;
       k++;
     }
-    if (w == 0 && lo <= 2 && hi > 2) out[0] = 2;
+    if (w == 0 && lo <= 2 && hi > 2 && cap) out[0] = 2;
   }
 }
 
 hipError_t launch_extract_range_u64(const PiEntry *tab, uint64_t lo, uint64_t hi, uint64_t *out,
-                                    hipStream_t s)
+                                    uint64_t cap, hipStream_t s)
 {
   if (hi <= lo) return hipSuccess;
   uint64_t nw = ((hi - 1) >> 6) - (lo >> 6) + 1;
   uint32_t g = (uint32_t)((nw + 255) / 256);
   if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(k_extract_range, dim3(g), dim3(256), 0, s, tab, lo, hi, out);
+  hipLaunchKernelGGL(k_extract_range, dim3(g), dim3(256), 0, s, tab, lo, hi, out, cap);
   return hipGetLastError();
 }

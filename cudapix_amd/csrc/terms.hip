@@ -167,33 +167,57 @@ hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s)
 }
 
 // -------------------------------------------------------- hard carry
-// For stage s (thread), walk the query's segments in order:
-//   carry += prefix * M[seg][s];  prefix += T[seg][s]
-// while the stage is active there (bact[seg] > s).
+// Hard-leaf carry of one stage s of one query per block:
+//   carry_s = sum over segments r (in order, while s is active in r) of
+//             msum_s(r) * (T_s(0) + ... + T_s(r-1))
+// with msum_s(r) the replicas' sum(mu) of the leaves of p_s in segment r and
+// T_s(r) its survivors before stage s.  Activity is a prefix of the segments
+// (bact is non-increasing in r), so the block scans 256 segments at a time
+// and stops after the first chunk that ends inactive.
 __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
 {
   __shared__ i128 red[4];
+  __shared__ unsigned long long wt[4];
+  __shared__ uint32_t stop;
   const uint32_t nd = *bd.n_fin;
+  const uint32_t t = threadIdx.x;
   for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
     const BlockDesc d = bd.fin_desc[di];
     const QueryDev &q = bd.queries[d.query];
-    const uint64_t s = d.begin + threadIdx.x;
+    const uint64_t s = d.begin;
     i128 carry = 0;
-    if (s < d.end) {
-      uint64_t prefix = 0;
-      for (uint64_t r = 0; r < q.hard_nseg; r++) {
+    unsigned long long base = 0;
+    for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += 256) {
+      const uint64_t r = r0 + t;
+      unsigned long long tv = 0;
+      long long msum = 0;
+      bool act = false;
+      if (r < q.hard_nseg) {
         const uint64_t row = q.hard_row0 + r;
-        if (bd.hard_bact[row] <= s) break;
-        const HardRowInfo ri = bd.rowinfo[row];
-        int64_t msum = 0;
-        for (uint32_t k = 0; k < ri.nrep; k++)
-          msum += bd.hard_M[q.m_off + ((uint64_t)ri.mrow + k) * q.nstage + s];
-        carry += (i128)prefix * msum;
-        prefix += bd.hard_T[q.tm_off + r * q.nstage + s];
+        if (bd.hard_bact[row] > s) {
+          act = true;
+          const HardRowInfo ri = bd.rowinfo[row];
+          tv = bd.hard_T[q.tm_off + ri.toff + s];
+          const int32_t *m = bd.hard_M + q.m_off + ri.moff + s;
+          for (uint32_t k = 0; k < ri.nrep; k++) msum += m[(uint64_t)k * ri.act];
+        }
       }
+      const unsigned long long incl = wave_incl_scan(tv);
+      if (lane_id() == CPX_WAVE - 1) wt[wave_id()] = incl;
+      __syncthreads();
+      unsigned long long pre = incl - tv, tot = 0;
+      for (uint32_t w = 0; w < 4; w++) {
+        tot += wt[w];
+        if (w < wave_id()) pre += wt[w];
+      }
+      carry += (i128)(base + pre) * msum;
+      base += tot;
+      if (t == 255) stop = act ? 0u : 1u;
+      __syncthreads();
+      if (stop) break;
     }
-    i128 t = block_sum_i128(carry, red);
-    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = t;
+    i128 tt = block_sum_i128(carry, red);
+    if (t == 0) bd.parts[q.part_off + d.part] = tt;
   }
 }
 

@@ -219,6 +219,12 @@ uint64_t cpx_request_bytes(cpx_ctx *ctx, cpx_u128 x);
  * roots against native __int128 on n pseudo-random cases; returns failures. */
 int cpx_selftest_arith(uint64_t seed, uint64_t n);
 
+/* ---- device self check (no reference counterpart) ------------------------ */
+/* Runs the segment kernel's 64-lane register scan and sum on nwaves pseudo-
+ * random wavefronts and compares each lane with a serial scan; *bad = number
+ * of wavefronts that differ. */
+int cpx_selftest_wave(cpx_ctx *ctx, uint64_t seed, uint32_t nwaves, uint32_t *bad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+def test_wave_register_scan(ctx):
+    # the DPP scan/sum used by every segment stage, lane by lane against a serial scan
+    for seed in (1, 2, 3):
+        print("case selftest_wave seed %d" % seed, flush=True)
+        assert ctx.selftest_wave(seed, 4096) == 0
+
+
 def test_count_primes_ranges(ctx, oracle):
     rnd = random.Random(3)
     cases = [(0, 10), (0, 2), (0, 3), (2, 3), (3, 100), (0, 10 ** 6), (10 ** 6, 10 ** 6 + 1000),
