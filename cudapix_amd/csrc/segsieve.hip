@@ -265,10 +265,13 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   for (;;) {
     const uint32_t cend = min(clen, sh.u.st.cend);
     // one-ahead prefetch of this thread's first candidate of the next leafy stage
+    // (and of x/p for that stage)
     uint32_t pf_k = hard_next_leafy<T, K>(sh, 0);
     int32_t pf = 0;
+    uint64_t pf_xp = 0;
     if (pf_k < cend) {
       const uint32_t cc = sh.u.st.ccnt[pf_k];
+      pf_xp = st[s0 + pf_k].xp;
       if (i0 < (cc & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc >> 31, i0);
     }
     uint32_t lw = 0;
@@ -285,7 +288,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
         const uint32_t cc = sh.u.st.ccnt[k], cb = sh.u.st.cbase[k];
         const bool sparse = cc >> 31;
         const uint32_t ncand = cc & 0x7fffffffu;
-        const uint64_t xp = st[s0 + k].xp;
+        const uint64_t xp = pf_xp;                            // x / p, fetched ahead
         int32_t smu = 0;
         int32_t v = pf;
         const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
@@ -307,9 +310,13 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
         pf_k = nk;
         if (nk < cend) {
           const uint32_t c2 = sh.u.st.ccnt[nk];
+          pf_xp = st[s0 + nk].xp;
           if (i0 < (c2 & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[nk], c2 >> 31, i0);
         }
-        if (smu) atomicAdd(&sh.u.st.msum[k], smu);
+        {
+          const int32_t wm = (int32_t)wave_sum((uint32_t)smu);   // one LDS add per wave
+          if (wm && lane == 0) atomicAdd(&sh.u.st.msum[k], wm);
+        }
         __syncthreads();                                      // readers done before cross-off
       }
       // ---- cross off the odd multiples of p in this thread's own bits
@@ -325,7 +332,9 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
           const uint32_t w = lds[addr];
           if (w & bit) { lds[addr] = w & ~bit; d++; }
         }
-        if (d) { atomicAdd(&sh.u.st.dec[k], d); cnt -= d; }
+        cnt -= d;
+        const uint32_t wd = wave_sum(d);                       // one LDS add per wave
+        if (wd && lane == 0) atomicAdd(&sh.u.st.dec[k], wd);
       }
     }
     nact = s0 + cend;
