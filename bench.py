@@ -112,6 +112,48 @@ def cpu_baseline(xs, budget_s=20.0):
                       % (done, t_sum, "" if done == k else ", untimed strata extrapolated x2")}
 
 
+def profile_run(args):
+    """One workload only, for profiler runs: `single` = pi(x) on a warm
+    context (default shape), `batch` = the 512-query batched step, `copy` = a
+    device-to-device copy of --copy-mib MiB (the byte-counter calibration)."""
+    if args.profile == "copy":
+        import torch
+        n = args.copy_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(args.reps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "copy", "bytes": n, "reps": args.reps}), flush=True)
+        return
+    import cudapix_amd as C
+    ctx = C.Context(device=0)
+    if args.profile == "single":
+        x = int(args.x)
+        ctx.pi(x)
+        ms = []
+        for _ in range(args.reps):
+            t0 = time.time()
+            v = ctx.pi(x)
+            ms.append((time.time() - t0) * 1000)
+        print(json.dumps({"profile": "single", "x": x, "pi": v, "ms": [round(m, 3) for m in ms]}),
+              flush=True)
+    else:
+        xs = load_json("queries_c3.json")["x"]
+        ref = ctx.pi_batch(xs)
+        ms = []
+        for _ in range(args.reps):
+            t0 = time.time()
+            if ctx.pi_batch(xs) != ref:
+                raise SystemExit("batched results changed between steps")
+            ms.append((time.time() - t0) * 1000)
+        print(json.dumps({"profile": "batch", "queries": len(xs), "ms": [round(m, 3) for m in ms]}),
+              flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,12 +166,21 @@ def main():
     ap.add_argument("--serve-n", type=int, default=400)
     ap.add_argument("--serve-lambdas", default="200,2000")
     ap.add_argument("--serve-xmax", type=float, default=1e16)
+    ap.add_argument("--profile", choices=["single", "batch", "copy"],
+                    help="run only one workload --reps times (for rocprofv3) and exit")
+    ap.add_argument("--x", type=float, default=1e16, help="x of --profile single")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=1024, help="bytes of --profile copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and this "
               "workload shards by query with no exchange (see DESIGN.md); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.profile:
+        profile_run(args)
+        return
 
     import cudapix_amd as C
     xs = load_json("queries_c3.json")["x"]

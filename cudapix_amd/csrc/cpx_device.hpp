@@ -27,6 +27,29 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
   return v;
 }
 
+// floor(a / m) for a < 2^64 and 2 <= m < 2^32 whose quotient is below 2^50,
+// given af = (double)a.  The hardware reciprocal is only about 2^-23 accurate;
+// two Newton steps (error squares each step) bring r within a few ulp of 1/m,
+// so af * r is within 0.375 of a/m (relative error < 1.5 * 2^-52, quotient
+// < 2^50).  Adding 2^52 in the same fma rounds it to an integer in the low
+// mantissa bits, within 0.875 of a/m: floor(a/m) or one above or below, and
+// one exact remainder test (a - q*m lies in [-m, 2m), exact in int64 whatever
+// the wrap of q*m) fixes it.  (One Newton step is not enough: near 2^50 the
+// estimate is then off by up to ~16.)
+__device__ __forceinline__ uint64_t cpx_div_leaf(uint64_t a, double af, uint32_t m)
+{
+  const double md = (double)m;
+  double r = __builtin_amdgcn_rcp(md);
+  r = __builtin_fma(__builtin_fma(-md, r, 1.0), r, r);
+  r = __builtin_fma(__builtin_fma(-md, r, 1.0), r, r);
+  const double est = __builtin_fma(af, r, 0x1p52);
+  uint64_t q = (uint64_t)__builtin_bit_cast(long long, est) & ((1ull << 52) - 1);
+  const int64_t rem = (int64_t)(a - q * (uint64_t)m);
+  if (rem < 0) q--;
+  else if (rem >= (int64_t)m) q++;
+  return q;
+}
+
 // Sum of 32-bit values over the wavefront, in every lane (DPP scan, then lane 63).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {

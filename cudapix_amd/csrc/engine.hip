@@ -345,12 +345,15 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     if (mask & TERM_EASY_A) ej1 = std::min<uint64_t>(ej1, d.pi_sqrtz);
     if (mask & TERM_EASY_B) ej0 = std::max<uint64_t>(ej0, d.pi_sqrtz);
     if ((mask & TERM_EASY) && ej1 > ej0) {
+      // work estimate only: q-count pi(min(y, x/p^2)) - j - 1, with x/p^2 in
+      // doubles and pi by a pointer that only moves down as p grows
       uint64_t j0 = ej0, work = 0;
+      const double fx = (double)h.x;
+      uint64_t k1 = d.a;                                  // pi(y)
       for (uint64_t j = ej0; j < ej1; j++) {
-        uint64_t p = c->h_primes[j];
-        u128 q2 = h.x / ((u128)p * p);
-        uint64_t qm = q2 > h.y ? h.y : (uint64_t)q2;
-        uint64_t k1 = host_pi(c, qm);
+        const double p = (double)c->h_primes[j];
+        const double qm = std::min((double)h.y, fx / (p * p));
+        while (k1 > 0 && (double)c->h_primes[k1 - 1] > qm) k1--;
         work += (k1 > j + 1 ? k1 - j - 1 : 0) + 64;
         if (work >= EASY_WORK || j + 1 == ej1) {
           pl.easy.push_back({(uint32_t)i, part++, j0, j + 1});
@@ -377,28 +380,31 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     d.mrow_base = pl.hard_mrows;
     d.m_off = pl.hard_mm;
     uint64_t mrows = 0, toff = 0, moff = 0;
+    uint64_t act = d.nstage;                             // non-increasing in the segment
     for (uint64_t s = 0; s < d.hard_nseg; s++) {
       const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
-      uint64_t xlo = ~0ull;
-      if (lo) {
-        u128 xl = h.x / lo;
-        xlo = (xl >> 64) ? ~0ull : (uint64_t)xl;
+      // active stages: p_b^2 <= x / lo, i.e. p_b^2 * lo <= x (the kernel's test)
+      while (act > 0) {
+        const u128 p = c->h_primes[h.c + act - 1];
+        if (p * p * (u128)lo <= h.x) break;
+        act--;
       }
-      // active stages: p_b^2 <= x / lo
-      uint64_t act = host_pi(c, std::min<uint64_t>(h.qrtx, lo ? cpx_isqrt128((u128)xlo) : h.qrtx));
-      act = act > h.c ? std::min<uint64_t>(act - h.c, d.nstage) : 0;
       double leaves = 0, leafy = 0;
       const bool est = sparse_run < 4;
-      for (uint32_t k = 0; est && k < act; k++) {
-        const uint64_t p = c->h_primes[h.c + k];
-        const uint64_t xp = xps[k];
-        uint64_t mhi = lo ? std::min<uint64_t>(h.y, xp / lo) : h.y;
-        uint64_t mlo = std::max<uint64_t>(xp / hi, h.y / p);
-        const bool sparse = p * p > h.y;
-        if (sparse) mlo = std::max<uint64_t>(mlo, p);
+      const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
+      const double fy = (double)h.y;
+      // (a sample of at most 32 stages, scaled: the estimate only sizes replicas)
+      const uint32_t kstep = act > 32 ? (uint32_t)((act + 31) / 32) : 1u;
+      for (uint32_t k = 0; est && k < act; k += kstep) {
+        const double p = (double)c->h_primes[h.c + k];
+        const double xp = (double)xps[k];
+        const double mhi = lo ? std::min(fy, std::floor(xp * inv_lo)) : fy;
+        double mlo = std::max(std::floor(xp * inv_hi), std::floor(fy / p));
+        const bool sparse = p * p > fy;
+        if (sparse) mlo = std::max(mlo, p);
         if (mhi > mlo) {
-          leafy += 1;
-          leaves += sparse ? (double)(mhi - mlo) / std::log((double)mhi) : 0.5 * (double)(mhi - mlo);
+          leafy += kstep;
+          leaves += kstep * (sparse ? (mhi - mlo) / std::log(mhi) : 0.5 * (mhi - mlo));
         }
       }
       if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
@@ -416,9 +422,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     pl.hard_mrows += mrows;
     pl.hard_tm += toff;
     pl.hard_mm += moff;
+    uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
     for (uint64_t s = 0; s < d.p2_nseg; s++) {
-      uint64_t hi = d.p2_e + (s + 1) * 2 * SB;
-      double nsp = (double)host_pi(c, std::min<uint64_t>(cpx_isqrt128(hi), c->prime_limit));
+      const double sq = std::min(std::sqrt((double)(d.p2_e + (s + 1) * 2 * SB)), (double)c->prime_limit);
+      while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
+      const double nsp = (double)nsp_i;
       double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
       tiles.push_back({cost, {(uint32_t)i, (uint16_t)TILE_P2, (uint16_t)0, s}});
     }
@@ -426,10 +434,18 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     pl.p2_segments += d.p2_nseg;
     pl.q[i] = d;
   }
-  std::stable_sort(tiles.begin(), tiles.end(),
-                   [](const TileCost &a, const TileCost &b) { return a.cost > b.cost; });
+  // longest first: counting sort on 1/8-octave buckets of the cost
+  // (descending, stable within a bucket), O(tiles)
+  const int NB = 512;
+  auto bucket = [](double cost) {
+    int b = (int)std::floor(std::log2(std::max(cost, 1.0)) * 8.0);
+    return std::min(std::max(b, 0), NB - 1);
+  };
+  std::vector<uint32_t> start(NB + 1, 0);
+  for (const TileCost &tc : tiles) start[NB - 1 - bucket(tc.cost) + 1]++;
+  for (int b = 0; b < NB; b++) start[b + 1] += start[b];
   pl.tiles.resize(tiles.size());
-  for (size_t i = 0; i < tiles.size(); i++) pl.tiles[i] = tiles[i].t;
+  for (const TileCost &tc : tiles) pl.tiles[start[NB - 1 - bucket(tc.cost)]++] = tc.t;
   return CPX_OK;
 }
 

@@ -77,8 +77,10 @@ struct cpx_ctx {
   void *h_res = nullptr;   size_t h_res_cap = 0;
   uint64_t generation = 0;                      // bumps on any reallocation
 
-  // segment-kernel shape
-  SegShape shape{64, 64};
+  // segment-kernel shape: 512 threads x 32 words (2^19-bit segments, two
+  // blocks per CU) measured fastest for single queries to 1e16 on MI355X;
+  // cpx_autotune may replace it for a batch
+  SegShape shape{512, 32};
   bool shape_tuned = false;
 
   // captured graph of the batch launch sequence

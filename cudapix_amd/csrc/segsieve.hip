@@ -62,18 +62,41 @@ struct SegShared {
   uint32_t tile;
 };
 
+// Each thread publishes, packed in one LDS word, its in-wave exclusive prefix
+// of the survivor counts (low PRE_BITS bits: at most 63 * 32K) and the count
+// of its first K/2 words (above), so that a count below any bit scans at most
+// K/2 of the owner's words.
+template <int K>
+struct CountPack {
+  static constexpr uint32_t PRE_BITS = K <= 32 ? 16 : (K <= 64 ? 17 : 18);
+  static constexpr uint32_t PRE_MASK = (1u << PRE_BITS) - 1;
+};
+
 // Publish the block scan of the per-thread counts: in-wave exclusive prefix
-// per thread and one total per wave.  Caller syncs afterwards.
+// and half count per thread (packed), one total per wave.  Caller syncs afterwards.
 template <int T, int K>
-__device__ __forceinline__ void publish_counts(uint32_t cnt, SegShared<T, K> &sh)
+__device__ __forceinline__ void publish_counts(uint32_t cnt, uint32_t cnt_lo, SegShared<T, K> &sh)
 {
   uint32_t incl = wave_incl_scan(cnt);
-  sh.ipre[threadIdx.x] = incl - cnt;
+  sh.ipre[threadIdx.x] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
   if (lane_id() == CPX_WAVE - 1) sh.wtot[wave_id()] = incl;
 }
 
+// Survivor count of the thread's first K/2 words.
+template <int K>
+__device__ __forceinline__ uint32_t own_count_lo(const uint32_t *lds, uint32_t t)
+{
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < (uint32_t)K / 2; j++) c += cpx_popc32(lds[own_addr(t, j, K)]);
+  return c;
+}
+
 // Number of survivors among the first `bit` bits of the segment
-// (0 <= bit <= 32*K*T), given the wave prefixes wp[] held in registers.
+// (0 <= bit <= 32*K*T), given the wave prefixes wp[] held in registers:
+// wave prefix + in-wave prefix of the owning thread + its words below the
+// bit, counted from the nearer half boundary (at most K/2 words, read four
+// at a time so that each group costs one LDS round trip).
 template <int T, int K>
 __device__ __forceinline__ uint32_t count_below(const uint32_t *lds, const SegShared<T, K> &sh,
                                                 const uint32_t (&wp)[T / CPX_WAVE], uint32_t bit)
@@ -85,11 +108,20 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t *lds, const SegSh
   uint32_t base = 0;
 #pragma unroll
   for (uint32_t w = 0; w < (uint32_t)(T / CPX_WAVE); w++) base = (owner / CPX_WAVE == w) ? wp[w] : base;
-  uint32_t c = base + sh.ipre[owner];
-  const uint32_t wix = jb >> 5;
-  const uint32_t a0 = owner * (K + 1);
-  for (uint32_t j = 0; j < wix; j++) c += cpx_popc32(lds[a0 + j]);
-  if (jb & 31u) c += cpx_popc32(lds[a0 + wix] & cpx_lowmask32(jb & 31u));
+  const uint32_t e = sh.ipre[owner];
+  uint32_t c = base + (e & CountPack<K>::PRE_MASK);
+  uint32_t w0 = 0;                                   // first word to scan
+  if (jb >= BPT / 2) { c += e >> CountPack<K>::PRE_BITS; w0 = K / 2; }
+  const uint32_t wix = jb >> 5;                      // words [w0, wix) are whole
+  const uint32_t *ow = lds + owner * (K + 1);
+  for (uint32_t g = w0; g < wix; g += 4) {
+    const uint32_t x0 = ow[g], x1 = ow[g + 1], x2 = ow[g + 2], x3 = ow[g + 3];
+    c += cpx_popc32(x0);
+    c += g + 1 < wix ? cpx_popc32(x1) : 0u;
+    c += g + 2 < wix ? cpx_popc32(x2) : 0u;
+    c += g + 3 < wix ? cpx_popc32(x3) : 0u;
+  }
+  if (jb & 31u) c += cpx_popc32(ow[wix] & cpx_lowmask32(jb & 31u));
   return c;
 }
 
@@ -247,9 +279,9 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
   if (t == 0) sh.u.st.cend = ~0u;
   __syncthreads();
-  uint32_t cnt = 0;
+  uint32_t cnt_lo = own_count_lo<K>(lds, t), cnt = cnt_lo;   // survivors: first half, all
 #pragma unroll 8
-  for (uint32_t j = 0; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
+  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
   {
     uint32_t ws = wave_sum(cnt);
     if (lane == 0) sh.wsum[wv] = (int32_t)ws;           // wtot is rewritten by the stages
@@ -281,7 +313,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
       if ((lw >> (k & 31)) & 1u) {
         // ---- leaves of p (state: crossed by p_1 .. p_{b-1}); needs every
         //      thread's words and counts, so the block meets here
-        publish_counts<T, K>(cnt, sh);
+        publish_counts<T, K>(cnt, cnt_lo, sh);
         __syncthreads();
         uint32_t wp[NW];
         (void)read_wave_prefix<T, K>(sh, wp);
@@ -289,6 +321,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
         const bool sparse = cc >> 31;
         const uint32_t ncand = cc & 0x7fffffffu;
         const uint64_t xp = pf_xp;                            // x / p, fetched ahead
+        const double xpf = (double)xp;
         int32_t smu = 0;
         int32_t v = pf;
         const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
@@ -303,7 +336,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
             if (av <= p) continue;                            // mu(m) = 0 or lpf(m) <= p
             m = 2 * (uint64_t)(cb - i) + 1;
           }
-          const uint64_t n = cpx_div_smallq(xp, m);         // lo <= n < hi
+          const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo <= n < hi (< 2^50)
           const uint32_t c = count_below<T, K>(lds, sh, wp, (uint32_t)((n - lo + 1) >> 1));
           if (v > 0) { acc += c; smu++; } else { acc -= c; smu--; }
         }
@@ -325,14 +358,15 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
         const uint32_t tb = cpx_fastmod32(t * BPT, p, sh.u.st.magic[k]);
         uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
         const uint32_t a0 = t * (K + 1);
-        uint32_t d = 0;
+        uint32_t d = 0, dlo = 0;
         for (; j < BPT; j += p) {
           const uint32_t addr = a0 + (j >> 5);
           const uint32_t bit = 1u << (j & 31u);
           const uint32_t w = lds[addr];
-          if (w & bit) { lds[addr] = w & ~bit; d++; }
+          if (w & bit) { lds[addr] = w & ~bit; d++; dlo += j < BPT / 2; }
         }
         cnt -= d;
+        cnt_lo -= dlo;
         const uint32_t wd = wave_sum(d);                       // one LDS add per wave
         if (wd && lane == 0) atomicAdd(&sh.u.st.dec[k], wd);
       }
@@ -402,10 +436,11 @@ __device__ __forceinline__ void p2_segment(const BatchDev &bd, const QueryDev &q
       atomicAnd(&lds[seg_addr((uint32_t)(j >> 5), K, T)], ~(1u << (j & 31)));
   }
   __syncthreads();
-  uint32_t cnt = 0;
+  const uint32_t cnt_lo = own_count_lo<K>(lds, t);
+  uint32_t cnt = cnt_lo;
 #pragma unroll 8
-  for (uint32_t j = 0; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
-  publish_counts<T, K>(cnt, sh);
+  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
+  publish_counts<T, K>(cnt, cnt_lo, sh);
   __syncthreads();
   uint32_t wp[NW];
   const uint32_t tot = read_wave_prefix<T, K>(sh, wp);
@@ -538,6 +573,18 @@ __global__ __launch_bounds__(CPX_WAVE) void k_selftest_wave(const uint32_t *__re
   const uint32_t incl = wave_incl_scan(a);
   const uint32_t sum = wave_sum(a);
   if (incl != ref || sum != tot) atomicAdd(bad, 1u);
+  // the leaf divide against the exact 64-bit divide: m in [2, 2^32), a with
+  // a / m < 2^50 (splitmix64 of the input word and the lane)
+  uint64_t h = ((uint64_t)a << 32) ^ ((uint64_t)blockIdx.x * CPX_WAVE + l) ^ 0x9e3779b97f4a7c15ull;
+  h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ull;
+  h = (h ^ (h >> 27)) * 0x94d049bb133111ebull;
+  h ^= h >> 31;
+  uint32_t m = (uint32_t)(h >> (32 + (a & 31)));
+  if (m < 2) m = 2;
+  uint64_t g = h * 0x2545f4914f6cdd1dull + l;
+  if (m < (1u << 14)) g %= ((uint64_t)m << 50);
+  if ((a & 7) == 0) g = ((uint64_t)m << 32) * ((uint64_t)1 << (a >> 27)) - 1 - (a & 1);  // near multiples
+  if (g / m < (1ull << 50) && cpx_div_leaf(g, (double)g, m) != g / m) atomicAdd(bad, 1u);
 }
 
 hipError_t launch_selftest_wave(const uint32_t *in, uint32_t n, uint32_t *bad, hipStream_t s)

@@ -283,14 +283,27 @@ __global__ __launch_bounds__(256) void k_combine(BatchDev bd)
       put_i128(res.p2, 0);
       res.pi_y = q.a;
       res.status = 1;
-    } else if (t == 0) {
+    }
+    // the term kernels' block partial sums, by term (ordinary | trivial U | easy | hard carry)
+    i128 ps0 = 0, psu = 0, psv = 0, psh = 0;
+    if (!q.small) {
       const i128 *pp = bd.parts + q.part_off;
-      i128 s0 = 0, u = 0, v = 0, hc = 0;
-      uint32_t k = 0;
-      for (uint32_t i = 0; i < q.n_ord_parts; i++) s0 += pp[k++];
-      for (uint32_t i = 0; i < q.n_triv_parts; i++) u += pp[k++];
-      for (uint32_t i = 0; i < q.n_easy_parts; i++) v += pp[k++];
-      for (uint32_t i = 0; i < q.n_hard_parts; i++) hc += pp[k++];
+      const uint32_t n0 = q.n_ord_parts, n1 = n0 + q.n_triv_parts, n2 = n1 + q.n_easy_parts,
+                     n3 = n2 + q.n_hard_parts;
+      for (uint32_t i = t; i < n3; i += blockDim.x) {
+        const i128 v = pp[i];
+        if (i < n0) ps0 += v;
+        else if (i < n1) psu += v;
+        else if (i < n2) psv += v;
+        else psh += v;
+      }
+    }
+    ps0 = block_sum_i128(ps0, red);
+    psu = block_sum_i128(psu, red);
+    psv = block_sum_i128(psv, red);
+    psh = block_sum_i128(psh, red);
+    if (t == 0 && !q.small) {
+      const i128 s0 = ps0, u = psu, v = psv, hc = psh;
       // S1 = C(pi(y) - pi(cbrt x), 2)
       i128 kk = (i128)q.a - (i128)q.pi_cbrt;
       i128 s1 = kk > 0 ? kk * (kk - 1) / 2 : 0;
