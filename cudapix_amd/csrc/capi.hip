@@ -652,7 +652,8 @@ int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, ui
 {
   CTX_GUARD(c);
   if (!xs || !n) return CPX_EINVAL;
-  static const SegShape cands[] = {{64, 32}, {64, 64}, {64, 128}, {128, 32}, {128, 64}, {256, 32}, {256, 64}, {512, 32}};
+  static const SegShape cands[] = {{64, 32}, {64, 64}, {64, 128}, {128, 32}, {128, 64}, {256, 32},
+                                   {256, 64}, {256, 128}, {512, 32}, {512, 64}};
   std::vector<HostQuery> hq(n);
   for (size_t i = 0; i < n; i++) {
     int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);

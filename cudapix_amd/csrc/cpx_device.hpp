@@ -130,6 +130,17 @@ __device__ __forceinline__ uint64_t first_odd_multiple_bit(uint64_t lo, uint32_t
   return (m - lo - 1) >> 1;
 }
 
+// first_odd_multiple_bit for max(lo + 1, from) / p < 2^50, with the
+// reciprocal divide in place of the 64-bit remainder.
+__device__ __forceinline__ uint64_t first_odd_multiple_bit_r(uint64_t lo, uint32_t p, uint64_t from)
+{
+  const uint64_t start = lo + 1 > from ? lo + 1 : from;
+  const uint64_t r = start - cpx_div_leaf(start, (double)start, p) * p;
+  uint64_t m = start + (r ? p - r : 0);
+  if (!(m & 1)) m += p;
+  return (m - lo - 1) >> 1;
+}
+
 // Cross off, in an interleaved LDS bitmap of `nbits` odd numbers starting at
 // integer lo (lo even), all odd multiples >= p^2 of the odd sieving primes
 // sp[i0 .. i1) with p^2 < lo + 2*nbits.  Small primes are shared by the whole

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <atomic>
+#include <thread>
 
 static double now_ms()
 {
@@ -94,11 +96,24 @@ uint64_t host_pi(const cpx_ctx *c, uint64_t v)
 // (src/main.cpp:128-160).
 static double alpha_model(u128 x)
 {
-  double lx = std::log10((double)x);
-  double a = 0.25 * lx - 1.0;
-  if (a < 1.0) a = 1.0;
-  if (a > 8.0) a = 8.0;
-  return a;
+  // best alpha per decade from a sweep over alpha in {1..32} at x = 1e10..1e16
+  // (one query, resident tables), linear in log10(x) between the points and
+  // 1.5 per decade beyond 1e16
+  static const double pts[][2] = {{11, 1.0}, {12, 2.0}, {13, 2.0}, {14, 2.5}, {15, 5.0}, {16, 6.0}};
+  const int np = (int)(sizeof(pts) / sizeof(pts[0]));
+  const double lx = std::log10((double)x);
+  double a;
+  if (lx <= pts[0][0]) {
+    a = pts[0][1];
+  } else if (lx >= pts[np - 1][0]) {
+    a = pts[np - 1][1] + 1.5 * (lx - pts[np - 1][0]);
+  } else {
+    int i = 1;
+    while (pts[i][0] < lx) i++;
+    const double f = (lx - pts[i - 1][0]) / (pts[i][0] - pts[i - 1][0]);
+    a = pts[i - 1][1] + f * (pts[i][1] - pts[i - 1][1]);
+  }
+  return std::min(a, 24.0);
 }
 
 uint64_t choose_y_host(const cpx_ctx *c, u128 x)
@@ -284,168 +299,242 @@ static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint
   (void)c;
 }
 
-struct TileCost { double cost; TileDev t; };
+// One query's share of the plan, with query-local offsets (rows, parts,
+// counters); build_plan places the shares.
+struct QPlan {
+  QueryDev d{};
+  std::vector<BlockDesc> ord, triv, easy, fin;
+  std::vector<HardRowInfo> rowinfo;
+  std::vector<TileDev> tiles;
+  std::vector<uint16_t> tbucket;     // cost bucket of each tile (longest first)
+  uint64_t parts = 0, tm = 0, mm = 0;
+};
 
-static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl)
+// Cost buckets: 1/8 octave of the tile's cost estimate.
+static const int PLAN_NB = 512;
+static inline uint16_t cost_bucket(double cost)
+{
+  int b = (int)std::floor(std::log2(std::max(cost, 1.0)) * 8.0);
+  return (uint16_t)std::min(std::max(b, 0), PLAN_NB - 1);
+}
+
+static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_t mask, QPlan &qp)
 {
   const uint64_t SB = c->shape.seg_bits();
   const uint32_t T = (uint32_t)c->shape.threads;
-  std::vector<TileCost> tiles;
-  pl.q.resize(hq.size());
-  for (size_t i = 0; i < hq.size(); i++) {
-    const HostQuery &h = hq[i];
-    QueryDev d{};
-    d.x_lo = (uint64_t)h.x;
-    d.x_hi = (uint64_t)(h.x >> 64);
-    d.part_off = pl.parts;
-    if (h.small) {
-      d.small = 1;
-      pl.q[i] = d;
-      continue;
+  QueryDev &d = qp.d;
+  d.x_lo = (uint64_t)h.x;
+  d.x_hi = (uint64_t)(h.x >> 64);
+  if (h.small) {
+    d.small = 1;
+    return;
+  }
+  d.y = h.y; d.z = h.z; d.sqrtx = h.sqrtx; d.cbrtx = h.cbrtx; d.qrtx = h.qrtx; d.sqrtz = h.sqrtz;
+  d.p2_zt = h.zt;
+  // the wheel pattern is word-aligned: segments start on a multiple of 64
+  d.p2_e = h.sqrtx & ~63ull;
+  d.c = h.c;
+  uint64_t B = host_pi(c, h.qrtx);
+  d.B = (uint32_t)B;
+  d.nstage = ((mask & TERM_HARD) && B > h.c) ? (uint32_t)(B - h.c) : 0;
+  d.a = (uint32_t)host_pi(c, h.y);
+  d.pi_cbrt = (uint32_t)host_pi(c, std::min(h.cbrtx, h.y));
+  d.pi_qrt = (uint32_t)B;
+  d.pi_sqrtz = (uint32_t)host_pi(c, h.sqrtz);
+  // hard segments over [0, z]; P2 segments over [p2_e, zt]
+  d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : 0;
+  d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? ceil_div(h.zt + 1 - d.p2_e, 2 * SB) : 0;
+  uint32_t part = 0;
+  // ordinary: odd n <= y
+  if (mask & TERM_ORD) {
+    uint64_t nodd = (h.y + 1) / 2;
+    for (uint64_t b = 0; b < nodd; b += ORD_CH)
+      qp.ord.push_back({i, part++, b, std::min(nodd, b + ORD_CH)});
+  }
+  d.n_ord_parts = part;
+  // trivial U: prime indices [pi(sqrt z), pi(cbrt x))
+  if ((mask & TERM_TRIV) && d.pi_cbrt > d.pi_sqrtz) {
+    for (uint64_t b = d.pi_sqrtz; b < d.pi_cbrt; b += TRIV_CH)
+      qp.triv.push_back({i, part++, b, std::min<uint64_t>(d.pi_cbrt, b + TRIV_CH)});
+  }
+  d.n_triv_parts = part - d.n_ord_parts;
+  // easy V: prime indices [pi(x^1/4), pi(cbrt x)), grouped by q-count
+  uint64_t ej0 = d.pi_qrt, ej1 = d.pi_cbrt;
+  if (mask & TERM_EASY_A) ej1 = std::min<uint64_t>(ej1, d.pi_sqrtz);
+  if (mask & TERM_EASY_B) ej0 = std::max<uint64_t>(ej0, d.pi_sqrtz);
+  if ((mask & TERM_EASY) && ej1 > ej0) {
+    // work estimate only: q-count pi(min(y, x/p^2)) - j - 1, with x/p^2 in
+    // doubles and pi by a pointer that only moves down as p grows
+    uint64_t j0 = ej0, work = 0;
+    const double fx = (double)h.x;
+    uint64_t k1 = d.a;                                  // pi(y)
+    for (uint64_t j = ej0; j < ej1; j++) {
+      const double p = (double)c->h_primes[j];
+      const double qm = std::min((double)h.y, fx / (p * p));
+      while (k1 > 0 && (double)c->h_primes[k1 - 1] > qm) k1--;
+      work += (k1 > j + 1 ? k1 - j - 1 : 0) + 64;
+      if (work >= EASY_WORK || j + 1 == ej1) {
+        qp.easy.push_back({i, part++, j0, j + 1});
+        j0 = j + 1;
+        work = 0;
+      }
     }
-    d.y = h.y; d.z = h.z; d.sqrtx = h.sqrtx; d.cbrtx = h.cbrtx; d.qrtx = h.qrtx; d.sqrtz = h.sqrtz;
-    d.p2_zt = h.zt;
-    // the wheel pattern is word-aligned: segments start on a multiple of 64
-    d.p2_e = h.sqrtx & ~63ull;
-    d.c = h.c;
-    uint64_t B = host_pi(c, h.qrtx);
-    d.B = (uint32_t)B;
-    d.nstage = ((mask & TERM_HARD) && B > h.c) ? (uint32_t)(B - h.c) : 0;
-    d.a = (uint32_t)host_pi(c, h.y);
-    d.pi_cbrt = (uint32_t)host_pi(c, std::min(h.cbrtx, h.y));
-    d.pi_qrt = (uint32_t)B;
-    d.pi_sqrtz = (uint32_t)host_pi(c, h.sqrtz);
+  }
+  d.n_easy_parts = part - d.n_ord_parts - d.n_triv_parts;
+  // hard carry: one block per stage
+  for (uint64_t b = 0; b < d.nstage; b++) qp.fin.push_back({i, part++, b, b + 1});
+  d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
+  qp.parts = part;
+  // hard tiles: active stages per segment and, for the low (leaf-dense)
+  // segments, an estimate of the leaves to evaluate; a segment with many
+  // leaves becomes several replica tiles that split its leaves.
+  std::vector<uint64_t> xps;
+  if (d.hard_nseg) {
+    xps.resize(d.nstage);
+    for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (uint64_t)(h.x / c->h_primes[h.c + k]);
+  }
+  const double leaf_target = 256.0 * T;                // leaf evaluations per replica
+  uint32_t sparse_run = 0;
+  uint64_t mrows = 0, toff = 0, moff = 0;
+  uint64_t act = d.nstage;                             // non-increasing in the segment
+  qp.rowinfo.reserve(d.hard_nseg);
+  qp.tiles.reserve(d.hard_nseg + d.p2_nseg);
+  qp.tbucket.reserve(d.hard_nseg + d.p2_nseg);
+  for (uint64_t s = 0; s < d.hard_nseg; s++) {
+    const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
+    // active stages: p_b^2 <= x / lo, i.e. p_b^2 * lo <= x (the kernel's test)
+    while (act > 0) {
+      const u128 p = c->h_primes[h.c + act - 1];
+      if (p * p * (u128)lo <= h.x) break;
+      act--;
+    }
+    double leaves = 0, leafy = 0;
+    const bool est = sparse_run < 4;
+    const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
+    const double fy = (double)h.y;
+    // (a sample of at most 32 stages, scaled: the estimate only sizes replicas)
+    const uint32_t kstep = act > 32 ? (uint32_t)((act + 31) / 32) : 1u;
+    for (uint32_t k = 0; est && k < act; k += kstep) {
+      const double p = (double)c->h_primes[h.c + k];
+      const double xp = (double)xps[k];
+      const double mhi = lo ? std::min(fy, std::floor(xp * inv_lo)) : fy;
+      double mlo = std::max(std::floor(xp * inv_hi), std::floor(fy / p));
+      const bool sparse = p * p > fy;
+      if (sparse) mlo = std::max(mlo, p);
+      if (mhi > mlo) {
+        leafy += kstep;
+        leaves += kstep * (sparse ? (mhi - mlo) / std::log(mhi) : 0.5 * (mhi - mlo));
+      }
+    }
+    if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
+    uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
+    qp.rowinfo.push_back({(uint32_t)mrows, nrep, (uint32_t)act, 0u, toff, moff});
+    mrows += nrep;
+    toff += act;
+    moff += (uint64_t)nrep * act;
+    const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
+                        2.0 * leaves / ((double)nrep * T);
+    const uint16_t bk = cost_bucket(cost);
+    for (uint32_t r = 0; r < nrep; r++) {
+      qp.tiles.push_back({i, (uint16_t)TILE_HARD, (uint16_t)r, s});
+      qp.tbucket.push_back(bk);
+    }
+  }
+  d.hard_nmrow = mrows;
+  qp.tm = toff;
+  qp.mm = moff;
+  uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
+  for (uint64_t s = 0; s < d.p2_nseg; s++) {
+    const double sq = std::min(std::sqrt((double)(d.p2_e + (s + 1) * 2 * SB)), (double)c->prime_limit);
+    while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
+    const double nsp = (double)nsp_i;
+    double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
+    qp.tiles.push_back({i, (uint16_t)TILE_P2, (uint16_t)0, s});
+    qp.tbucket.push_back(cost_bucket(cost));
+  }
+}
+
+// Run f(k) for k in [0, n) on up to `threads` host threads (inline for one).
+template <typename F>
+static void parallel_for(size_t n, unsigned threads, F f)
+{
+  if (n == 0) return;
+  threads = (unsigned)std::min<size_t>(std::max(1u, threads), n);
+  if (threads == 1) {
+    for (size_t k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
+static unsigned plan_threads(size_t nq)
+{
+  if (nq < 8) return 1;
+  unsigned hc = std::thread::hardware_concurrency();
+  return std::min<unsigned>(hc ? hc : 4, 16);
+}
+
+static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl)
+{
+  const size_t nq = hq.size();
+  std::vector<QPlan> qps(nq);
+  const unsigned nt = plan_threads(nq);
+  parallel_for(nq, nt, [&](size_t i) { plan_query(c, hq[i], (uint32_t)i, mask, qps[i]); });
+  // place the shares: global offsets of every query's rows, parts and counters
+  struct Off { size_t ord, triv, easy, fin, rows, tiles; };
+  std::vector<Off> off(nq);
+  size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, nrows = 0, ntiles = 0;
+  pl.q.resize(nq);
+  for (size_t i = 0; i < nq; i++) {
+    QPlan &qp = qps[i];
+    QueryDev &d = qp.d;
+    d.part_off = pl.parts;
     d.stage_off = pl.stage_total;
-    pl.stage_total += d.nstage;
-    // hard segments over [0, z]
-    d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : 0;
     d.hard_row0 = pl.hard_rows;
     d.tm_off = pl.hard_tm;
-    pl.hard_rows += d.hard_nseg;
-    // P2 segments over [p2_e, zt]
-    d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? ceil_div(h.zt + 1 - d.p2_e, 2 * SB) : 0;
     d.p2_row0 = pl.p2_rows;
-    pl.p2_rows += d.p2_nseg;
-    uint32_t part = 0;
-    // ordinary: odd n <= y
-    if (mask & TERM_ORD) {
-      uint64_t nodd = (h.y + 1) / 2;
-      for (uint64_t b = 0; b < nodd; b += ORD_CH)
-        pl.ord.push_back({(uint32_t)i, part++, b, std::min(nodd, b + ORD_CH)});
-    }
-    d.n_ord_parts = part;
-    // trivial U: prime indices [pi(sqrt z), pi(cbrt x))
-    if ((mask & TERM_TRIV) && d.pi_cbrt > d.pi_sqrtz) {
-      for (uint64_t b = d.pi_sqrtz; b < d.pi_cbrt; b += TRIV_CH)
-        pl.triv.push_back({(uint32_t)i, part++, b, std::min<uint64_t>(d.pi_cbrt, b + TRIV_CH)});
-    }
-    d.n_triv_parts = part - d.n_ord_parts;
-    // easy V: prime indices [pi(x^1/4), pi(cbrt x)), grouped by q-count
-    uint64_t ej0 = d.pi_qrt, ej1 = d.pi_cbrt;
-    if (mask & TERM_EASY_A) ej1 = std::min<uint64_t>(ej1, d.pi_sqrtz);
-    if (mask & TERM_EASY_B) ej0 = std::max<uint64_t>(ej0, d.pi_sqrtz);
-    if ((mask & TERM_EASY) && ej1 > ej0) {
-      // work estimate only: q-count pi(min(y, x/p^2)) - j - 1, with x/p^2 in
-      // doubles and pi by a pointer that only moves down as p grows
-      uint64_t j0 = ej0, work = 0;
-      const double fx = (double)h.x;
-      uint64_t k1 = d.a;                                  // pi(y)
-      for (uint64_t j = ej0; j < ej1; j++) {
-        const double p = (double)c->h_primes[j];
-        const double qm = std::min((double)h.y, fx / (p * p));
-        while (k1 > 0 && (double)c->h_primes[k1 - 1] > qm) k1--;
-        work += (k1 > j + 1 ? k1 - j - 1 : 0) + 64;
-        if (work >= EASY_WORK || j + 1 == ej1) {
-          pl.easy.push_back({(uint32_t)i, part++, j0, j + 1});
-          j0 = j + 1;
-          work = 0;
-        }
-      }
-    }
-    d.n_easy_parts = part - d.n_ord_parts - d.n_triv_parts;
-    // hard carry: one block per stage
-    for (uint64_t b = 0; b < d.nstage; b++) pl.fin.push_back({(uint32_t)i, part++, b, b + 1});
-    d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
-    pl.parts += part;
-    // hard tiles: active stages per segment and, for the low (leaf-dense)
-    // segments, an estimate of the leaves to evaluate; a segment with many
-    // leaves becomes several replica tiles that split its leaves.
-    std::vector<uint64_t> xps;
-    if (d.hard_nseg) {
-      xps.resize(d.nstage);
-      for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (uint64_t)(h.x / c->h_primes[h.c + k]);
-    }
-    const double leaf_target = 256.0 * T;                // leaf evaluations per replica
-    uint32_t sparse_run = 0;
     d.mrow_base = pl.hard_mrows;
     d.m_off = pl.hard_mm;
-    uint64_t mrows = 0, toff = 0, moff = 0;
-    uint64_t act = d.nstage;                             // non-increasing in the segment
-    for (uint64_t s = 0; s < d.hard_nseg; s++) {
-      const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
-      // active stages: p_b^2 <= x / lo, i.e. p_b^2 * lo <= x (the kernel's test)
-      while (act > 0) {
-        const u128 p = c->h_primes[h.c + act - 1];
-        if (p * p * (u128)lo <= h.x) break;
-        act--;
-      }
-      double leaves = 0, leafy = 0;
-      const bool est = sparse_run < 4;
-      const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
-      const double fy = (double)h.y;
-      // (a sample of at most 32 stages, scaled: the estimate only sizes replicas)
-      const uint32_t kstep = act > 32 ? (uint32_t)((act + 31) / 32) : 1u;
-      for (uint32_t k = 0; est && k < act; k += kstep) {
-        const double p = (double)c->h_primes[h.c + k];
-        const double xp = (double)xps[k];
-        const double mhi = lo ? std::min(fy, std::floor(xp * inv_lo)) : fy;
-        double mlo = std::max(std::floor(xp * inv_hi), std::floor(fy / p));
-        const bool sparse = p * p > fy;
-        if (sparse) mlo = std::max(mlo, p);
-        if (mhi > mlo) {
-          leafy += kstep;
-          leaves += kstep * (sparse ? (mhi - mlo) / std::log(mhi) : 0.5 * (mhi - mlo));
-        }
-      }
-      if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
-      uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
-      pl.rowinfo.push_back({(uint32_t)mrows, nrep, (uint32_t)act, 0u, toff, moff});
-      mrows += nrep;
-      toff += act;
-      moff += (uint64_t)nrep * act;
-      const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
-                          2.0 * leaves / ((double)nrep * T);
-      for (uint32_t r = 0; r < nrep; r++)
-        tiles.push_back({cost, {(uint32_t)i, (uint16_t)TILE_HARD, (uint16_t)r, s}});
-    }
-    d.hard_nmrow = mrows;
-    pl.hard_mrows += mrows;
-    pl.hard_tm += toff;
-    pl.hard_mm += moff;
-    uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
-    for (uint64_t s = 0; s < d.p2_nseg; s++) {
-      const double sq = std::min(std::sqrt((double)(d.p2_e + (s + 1) * 2 * SB)), (double)c->prime_limit);
-      while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
-      const double nsp = (double)nsp_i;
-      double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
-      tiles.push_back({cost, {(uint32_t)i, (uint16_t)TILE_P2, (uint16_t)0, s}});
-    }
+    pl.parts += qp.parts;
+    pl.stage_total += d.nstage;
+    pl.hard_rows += d.hard_nseg;
+    pl.hard_tm += qp.tm;
+    pl.p2_rows += d.p2_nseg;
+    pl.hard_mrows += d.hard_nmrow;
+    pl.hard_mm += qp.mm;
     pl.hard_segments += d.hard_nseg;
     pl.p2_segments += d.p2_nseg;
     pl.q[i] = d;
+    off[i] = {nord, ntriv, neasy, nfin, nrows, ntiles};
+    nord += qp.ord.size(); ntriv += qp.triv.size(); neasy += qp.easy.size();
+    nfin += qp.fin.size(); nrows += qp.rowinfo.size(); ntiles += qp.tiles.size();
   }
-  // longest first: counting sort on 1/8-octave buckets of the cost
-  // (descending, stable within a bucket), O(tiles)
-  const int NB = 512;
-  auto bucket = [](double cost) {
-    int b = (int)std::floor(std::log2(std::max(cost, 1.0)) * 8.0);
-    return std::min(std::max(b, 0), NB - 1);
-  };
-  std::vector<uint32_t> start(NB + 1, 0);
-  for (const TileCost &tc : tiles) start[NB - 1 - bucket(tc.cost) + 1]++;
-  for (int b = 0; b < NB; b++) start[b + 1] += start[b];
-  pl.tiles.resize(tiles.size());
-  for (const TileCost &tc : tiles) pl.tiles[start[NB - 1 - bucket(tc.cost)]++] = tc.t;
+  pl.ord.resize(nord); pl.triv.resize(ntriv); pl.easy.resize(neasy);
+  pl.fin.resize(nfin); pl.rowinfo.resize(nrows);
+  // longest first: counting sort on the cost buckets (descending, stable in
+  // query order within a bucket), O(tiles)
+  std::vector<uint32_t> start(PLAN_NB + 1, 0);
+  for (const QPlan &qp : qps)
+    for (uint16_t b : qp.tbucket) start[PLAN_NB - 1 - b + 1]++;
+  for (int b = 0; b < PLAN_NB; b++) start[b + 1] += start[b];
+  pl.tiles.resize(ntiles);
+  for (const QPlan &qp : qps)
+    for (size_t k = 0; k < qp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - qp.tbucket[k]]++] = qp.tiles[k];
+  parallel_for(nq, nt, [&](size_t i) {
+    const QPlan &qp = qps[i];
+    std::copy(qp.ord.begin(), qp.ord.end(), pl.ord.begin() + off[i].ord);
+    std::copy(qp.triv.begin(), qp.triv.end(), pl.triv.begin() + off[i].triv);
+    std::copy(qp.easy.begin(), qp.easy.end(), pl.easy.begin() + off[i].easy);
+    std::copy(qp.fin.begin(), qp.fin.end(), pl.fin.begin() + off[i].fin);
+    std::copy(qp.rowinfo.begin(), qp.rowinfo.end(), pl.rowinfo.begin() + off[i].rows);
+  });
   return CPX_OK;
 }
 

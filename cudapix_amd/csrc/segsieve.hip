@@ -36,7 +36,7 @@
 #include "cpx_device.hpp"
 
 #define P2_MAX_COOP 512    // LDS slots for shared-prime offsets
-#define HARD_SCH 256       // stages per chunk of LDS stage counters
+#define HARD_SCH 128       // stages per chunk of the hard stage table
 
 template <int T, int K>
 struct SegShared {
@@ -46,15 +46,22 @@ struct SegShared {
   unsigned long long red_u[T / CPX_WAVE];
   long long red_s[T / CPX_WAVE];
   union {
-    uint32_t offs[P2_MAX_COOP];  // P2: first multiple of each block-shared prime
-    struct {                     // hard: stage table of the current chunk
-      uint32_t dec[HARD_SCH];    // survivors removed by each stage
-      int32_t msum[HARD_SCH];    // sum of mu of each stage's leaves (this replica)
-      uint64_t magic[HARD_SCH];  // ceil(2^64 / p)
+    struct {                     // P2: the block-shared (small) sieving primes
+      uint32_t offs[P2_MAX_COOP];  // first odd multiple >= max(s0 + 1, p^2), as a bit
+      uint32_t p[P2_MAX_COOP];
+      float rinv[P2_MAX_COOP];     // 1 / p
+    } p2;
+    struct {                     // hard: stage table and per-wave counters of the chunk
+      uint64_t xp[HARD_SCH];     // floor(x / p)
       uint32_t p[HARD_SCH];
+      float rinv[HARD_SCH];      // 1 / p
       uint32_t oseg[HARD_SCH];   // first odd multiple of p in the segment (bit offset)
       uint32_t cbase[HARD_SCH];  // leaf candidates: mtop/2 (dense) or first prime index
-      uint32_t ccnt[HARD_SCH];   // number of candidates | 2^31 if sparse (p^2 > y)
+      uint32_t ccnt[HARD_SCH];   // candidates | HARD_WIDE | HARD_SPARSE
+      int32_t wm[HARD_SCH][T / CPX_WAVE];    // sum of mu of each wave's leaves per stage
+      uint16_t dw[HARD_SCH][T / CPX_WAVE];   // survivors each wave removes per stage
+      uint16_t wb[HARD_SCH][T / CPX_WAVE];   // first candidate of each wave (non-wide stages)
+      uint32_t c0[T / CPX_WAVE]; // each wave's survivors at the chunk start
       uint32_t lmask[HARD_SCH / 32];   // stages with candidates
       uint32_t cend;             // first inactive stage of the chunk (or ~0)
     } st;
@@ -134,35 +141,6 @@ __device__ __forceinline__ uint32_t read_wave_prefix(const SegShared<T, K> &sh, 
   return run;
 }
 
-// Write the counters of stages [c0, c1) of the current chunk: T_b (replica 0
-// only), sum(mu) of this replica; then clear the chunk.  All threads call;
-// `running` is kept by wave 0.
-template <int T, int K>
-__device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uint32_t c1,
-                                           uint32_t &running, uint32_t *outT, int32_t *outM,
-                                           bool write_T)
-{
-  __syncthreads();
-  if (wave_id() == 0) {                 // wave 0 scans the removals 64 stages at a time
-    const uint32_t lane = lane_id();
-    for (uint32_t k0 = c0; k0 < c1; k0 += CPX_WAVE) {
-      const uint32_t k = k0 + lane;
-      const uint32_t d = k < c1 ? sh.u.st.dec[k - c0] : 0u;
-      const uint32_t incl = wave_incl_scan(d);
-      if (k < c1) {
-        if (write_T) outT[k] = running - (incl - d);
-        outM[k] = sh.u.st.msum[k - c0];
-      }
-      running -= __shfl(incl, CPX_WAVE - 1);
-    }
-  }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < HARD_SCH; k += T) { sh.u.st.dec[k] = 0; sh.u.st.msum[k] = 0; }
-  if (threadIdx.x < HARD_SCH / 32) sh.u.st.lmask[threadIdx.x] = 0;
-  if (threadIdx.x == 0) sh.u.st.cend = ~0u;
-  __syncthreads();
-}
-
 // Leaf candidate i of chunk stage k: dense stages read mu*lpf of m = mtop - 2i
 // (index mtop/2 - i), sparse stages read the i-th prime above mlo.
 __device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase, bool sparse,
@@ -171,9 +149,21 @@ __device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase
   return sparse ? (int32_t)bd.primes[cbase + i] : bd.mlpf[cbase - i];
 }
 
+// ccnt word of the stage table: number of candidates (bits 0-29), "wide"
+// (bit 30: 2^16 candidates or more, so the per-wave boundaries are not
+// tabulated) and "sparse" (bit 31: p^2 > y, candidates are primes).
+#define HARD_NC_MASK 0x3fffffffu
+#define HARD_WIDE 0x40000000u
+#define HARD_SPARSE 0x80000000u
+
 // Stage table of one chunk: for its stages k < clen (stage s0 + k) the
-// threads write p, the Lemire constant, the first odd multiple of p in the
-// segment (bit offset), the leaf candidate range and a leafy bit; the first
+// threads write p, 1/p, x/p, the first odd multiple of p in the segment (bit
+// offset), the leaf candidate range, a leafy bit, and for every wave w > 0
+// the boundary b_w of its candidates (wave w owns the leaves with n = x/(p m)
+// in [lo_w, lo_w + 4*32K*64)): for dense stages (m = mtop - 2i, n grows with
+// i) b_w counts the candidates with n < lo_w and is the wave's first index;
+// for sparse stages (m = the i-th prime, n falls with i) it counts those with
+// n >= lo_w and is the wave's end.  Clears the per-wave sums of mu; the first
 // inactive stage (p^2 > x/lo) lowers st.cend.  Caller syncs before and after.
 template <int T, int K>
 __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const QueryDev &q,
@@ -182,7 +172,10 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const Query
                                                  double inv_lo, double inv_hi,
                                                  SegShared<T, K> &sh)
 {
+  constexpr uint32_t NW = T / CPX_WAVE;
+  constexpr uint64_t WB = (uint64_t)CPX_WAVE * 32 * K;
   const uint64_t y = q.y;
+  for (uint32_t k = threadIdx.x; k < clen * NW; k += T) (&sh.u.st.wm[0][0])[k] = 0;
   for (uint32_t k = threadIdx.x; k < clen; k += T) {
     const StageDev sd = st[s0 + k];
     const uint64_t p = sd.p, xp = sd.xp;
@@ -191,7 +184,8 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const Query
       continue;
     }
     sh.u.st.p[k] = (uint32_t)p;
-    sh.u.st.magic[k] = sd.magic;
+    sh.u.st.rinv[k] = 1.0f / (float)p;
+    sh.u.st.xp[k] = xp;
     // first odd multiple of p at or above lo + 1, as a bit offset (< p)
     const uint64_t start = lo + 1;
     const uint64_t rr = start - cpx_div_recip(start, p, sd.inv_p) * p;
@@ -224,8 +218,30 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const Query
       }
     }
     sh.u.st.cbase[k] = cbase;
-    sh.u.st.ccnt[k] = ncand | (sparse ? 0x80000000u : 0u);
+    sh.u.st.ccnt[k] = ncand | (sparse ? HARD_SPARSE : 0u) | (ncand > 0xffffu ? HARD_WIDE : 0u);
     if (ncand) atomicOr(&sh.u.st.lmask[k >> 5], 1u << (k & 31));
+  }
+  if (NW == 1) return;
+  __syncthreads();
+  // n >= lo_w <=> m <= M = floor(x/p / lo_w)
+  const uint32_t cend = min(clen, sh.u.st.cend);
+  for (uint32_t idx = threadIdx.x; idx < cend * NW; idx += T) {
+    const uint32_t k = idx / NW, w = idx % NW;
+    const uint32_t cc = sh.u.st.ccnt[k];
+    if (w == 0 || !(cc & HARD_NC_MASK) || (cc & HARD_WIDE)) continue;
+    const uint32_t ncand = cc & HARD_NC_MASK, cb = sh.u.st.cbase[k];
+    const uint64_t xp = sh.u.st.xp[k];
+    const uint64_t lo_w = lo + 2 * (uint64_t)w * WB;          // > 0
+    const uint64_t M = cpx_div_recip(xp, lo_w, 1.0 / (double)lo_w);
+    uint64_t b;
+    if (!(cc & HARD_SPARSE)) {
+      const uint64_t mtop = 2 * (uint64_t)cb + 1;            // m = mtop - 2i
+      b = mtop > M ? (mtop - M + 1) / 2 : 0;
+    } else {
+      const uint64_t pm = pi_lookup(bd.pitab, M < y ? M : y);
+      b = pm > cb ? pm - cb : 0;
+    }
+    sh.u.st.wb[k][w] = (uint16_t)(b < ncand ? b : ncand);
   }
 }
 
@@ -241,12 +257,143 @@ __device__ __forceinline__ uint32_t hard_next_leafy(const SegShared<T, K> &sh, u
   return HARD_SCH;
 }
 
+// This wave's candidate index range [ia, ib) of chunk stage k.  Stages with
+// 2^16 candidates or more derive it here: n = x/(p m) in [lo_w, hi_w) <=>
+// floor(xp / hi_w) < m <= floor(xp / lo_w).
+template <int T, int K>
+__device__ __forceinline__ void hard_wave_range(const BatchDev &bd, const QueryDev &q,
+                                                const SegShared<T, K> &sh, uint32_t k, uint32_t wv,
+                                                uint64_t lo_w, uint64_t hi_w, double inv_lo_w,
+                                                double inv_hi_w, uint32_t &ia, uint32_t &ib)
+{
+  constexpr uint32_t NW = T / CPX_WAVE;
+  const uint32_t cc = sh.u.st.ccnt[k], ncand = cc & HARD_NC_MASK;
+  if (!(cc & HARD_WIDE)) {
+    if (!(cc & HARD_SPARSE)) {               // n grows with the index
+      ia = wv ? sh.u.st.wb[k][wv] : 0u;
+      ib = wv + 1 < NW ? sh.u.st.wb[k][wv + 1] : ncand;
+    } else {                                 // n falls with the index
+      ia = wv + 1 < NW ? sh.u.st.wb[k][wv + 1] : 0u;
+      ib = wv ? sh.u.st.wb[k][wv] : ncand;
+    }
+    return;
+  }
+  const uint32_t cb = sh.u.st.cbase[k];
+  const uint64_t xp = sh.u.st.xp[k];
+  const uint64_t Mlo = lo_w ? cpx_div_recip(xp, lo_w, inv_lo_w) : ~0ull;
+  const uint64_t Mhi = cpx_div_recip(xp, hi_w, inv_hi_w);
+  if (!(cc & HARD_SPARSE)) {                                 // m = mtop - 2i
+    const uint64_t mtop = 2 * (uint64_t)cb + 1;
+    const uint64_t a = mtop > Mlo ? (mtop - Mlo + 1) / 2 : 0;
+    const uint64_t b = mtop > Mhi ? (mtop - Mhi + 1) / 2 : 0;
+    ia = (uint32_t)min(a, (uint64_t)ncand);
+    ib = (uint32_t)min(b, (uint64_t)ncand);
+  } else {                                                   // primes[cb + i]
+    const uint64_t y = q.y;
+    const uint64_t pl = pi_lookup(bd.pitab, Mhi < y ? Mhi : y);
+    const uint64_t ph = pi_lookup(bd.pitab, Mlo < y ? Mlo : y);
+    ia = (uint32_t)(pl > cb ? min(pl - cb, (uint64_t)ncand) : 0);
+    ib = (uint32_t)(ph > cb ? min(ph - cb, (uint64_t)ncand) : 0);
+  }
+}
+
+// Survivors among the first `gw` bits (0 <= gw <= 64 * 32K) of this wave's
+// part of the segment: in-wave prefix of the owning lane + its words below,
+// from the packed prefix/half-count words the wave published.
+template <int T, int K>
+__device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const SegShared<T, K> &sh,
+                                                  uint32_t wv, uint32_t gw)
+{
+  constexpr uint32_t BPT = 32 * K;
+  uint32_t ol = gw / BPT;
+  uint32_t jb = gw - ol * BPT;
+  if (ol >= (uint32_t)CPX_WAVE) { ol = CPX_WAVE - 1; jb = BPT; }
+  const uint32_t owner = wv * CPX_WAVE + ol;
+  const uint32_t e = sh.ipre[owner];
+  uint32_t c = e & CountPack<K>::PRE_MASK;
+  uint32_t w0 = 0;
+  if (jb >= BPT / 2) { c += e >> CountPack<K>::PRE_BITS; w0 = K / 2; }
+  const uint32_t wix = jb >> 5;
+  const uint32_t *ow = lds + owner * (K + 1);
+  for (uint32_t g = w0; g < wix; g += 4) {
+    const uint32_t x0 = ow[g], x1 = ow[g + 1], x2 = ow[g + 2], x3 = ow[g + 3];
+    c += cpx_popc32(x0);
+    c += g + 1 < wix ? cpx_popc32(x1) : 0u;
+    c += g + 2 < wix ? cpx_popc32(x2) : 0u;
+    c += g + 3 < wix ? cpx_popc32(x3) : 0u;
+  }
+  if (jb & 31u) c += cpx_popc32(ow[wix] & cpx_lowmask32(jb & 31u));
+  return c;
+}
+
+// End of a chunk of stages [c0, c1) (all threads call; `running` is kept by
+// wave 0): writes T_b = survivors of the segment before stage b (replica 0)
+// and this replica's sum of mu per stage, and adds to acc the part of every
+// leaf's phi that the in-wave count left out -- the survivors of the earlier
+// waves at that stage:
+//   sum_k sum_w wm[k][w] * (A_w - sum_{j<k} sum_{w'<w} dw[j][w'])
+// with A_w the earlier waves' survivors at the chunk start.  Then resets the
+// chunk's leafy mask and end marker.
+template <int T, int K>
+__device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uint32_t c1,
+                                           uint32_t &running, uint32_t *outT, int32_t *outM,
+                                           bool write_T, int64_t &acc)
+{
+  constexpr uint32_t NW = T / CPX_WAVE;
+  const uint32_t lane = lane_id(), wv = wave_id();
+  const uint32_t n = c1 - c0;
+  __syncthreads();
+  if (NW > 1) {
+    uint32_t A = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; w++) A += w < wv ? sh.u.st.c0[w] : 0u;
+    uint32_t carry = 0;
+    int64_t corr = 0;
+    for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
+      const uint32_t k = k0 + lane;
+      uint32_t G = 0;
+      int32_t m = 0;
+      if (k < n) {
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) G += w < wv ? (uint32_t)sh.u.st.dw[k][w] : 0u;
+        m = sh.u.st.wm[k][wv];
+      }
+      const uint32_t incl = wave_incl_scan(G);
+      corr += (int64_t)m * ((int64_t)A - (int64_t)(carry + incl - G));
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
+    }
+    acc += corr;
+  }
+  if (wv == 0) {                       // T_b and the sum of mu per stage
+    for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
+      const uint32_t k = k0 + lane;
+      uint32_t d = 0;
+      int32_t ms = 0;
+      if (k < n) {
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) { d += sh.u.st.dw[k][w]; ms += sh.u.st.wm[k][w]; }
+      }
+      const uint32_t incl = wave_incl_scan(d);
+      if (k < n) {
+        if (write_T) outT[c0 + k] = running - (incl - d);
+        outM[c0 + k] = ms;
+      }
+      running -= (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < HARD_SCH / 32) sh.u.st.lmask[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.u.st.cend = ~0u;
+  __syncthreads();
+}
+
 template <int T, int K>
 __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev &q,
                                              const TileDev &tile, uint32_t *lds, SegShared<T, K> &sh)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
   constexpr uint32_t BPT = 32 * K;
+  constexpr uint32_t WB = CPX_WAVE * BPT;                // bits per wave
   const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
   const uint64_t SB = (uint64_t)BPT * T;
   const uint64_t seg = tile.seg;
@@ -264,7 +411,12 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   uint32_t *outT = bd.hard_T + q.tm_off + ri.toff;
   int32_t *outM = bd.hard_M + q.m_off + ri.moff + (uint64_t)rep * ri.act;
   const StageDev *st = bd.stages + q.stage_off;
-  const uint32_t i0 = rep * T + t, step = T * nrep;     // this thread's leaf candidates
+  // this wave's integers [lo_w, hi_w); this lane's first bit, for t*32K mod p
+  const uint64_t lo_w = lo + 2 * (uint64_t)wv * WB, hi_w = lo_w + 2 * (uint64_t)WB;
+  const double inv_lo_w = lo_w ? 1.0 / (double)lo_w : 0.0, inv_hi_w = 1.0 / (double)hi_w;
+  const uint32_t b0 = t * BPT;                          // < 2^20
+  const float fb0 = (float)b0;
+  const uint32_t cstep = CPX_WAVE * nrep;               // candidate stride of a lane
 
   // stage b is active while p_b^2 <= x / lo (all leaves of p_b have n < x/p_b^2)
   uint64_t xlo = ~0ull;
@@ -275,7 +427,6 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
 
   seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
-  for (uint32_t k = t; k < HARD_SCH; k += T) { sh.u.st.dec[k] = 0; sh.u.st.msum[k] = 0; }
   if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
   if (t == 0) sh.u.st.cend = ~0u;
   __syncthreads();
@@ -284,7 +435,7 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
   {
     uint32_t ws = wave_sum(cnt);
-    if (lane == 0) sh.wsum[wv] = (int32_t)ws;           // wtot is rewritten by the stages
+    if (lane == 0) sh.wsum[wv] = (int32_t)ws;
   }
   uint32_t s0 = 0, clen = min((uint32_t)HARD_SCH, nst);
   hard_chunk_table<T, K>(bd, q, st, s0, clen, lo, hi, xlo, inv_lo, inv_hi, sh);
@@ -296,83 +447,118 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
   uint32_t nact = 0;                                    // active stages of the segment
   for (;;) {
     const uint32_t cend = min(clen, sh.u.st.cend);
-    // one-ahead prefetch of this thread's first candidate of the next leafy stage
-    // (and of x/p for that stage)
+    {
+      const uint32_t ws = wave_sum(cnt);                // this wave's survivors at the chunk start
+      if (lane == 0) sh.u.st.c0[wv] = ws;
+    }
+    // one-ahead prefetch of this lane's first candidate of the next leafy stage
     uint32_t pf_k = hard_next_leafy<T, K>(sh, 0);
     int32_t pf = 0;
-    uint64_t pf_xp = 0;
     if (pf_k < cend) {
+      uint32_t ia, ib;
+      hard_wave_range<T, K>(bd, q, sh, pf_k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
       const uint32_t cc = sh.u.st.ccnt[pf_k];
-      pf_xp = st[s0 + pf_k].xp;
-      if (i0 < (cc & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc >> 31, i0);
+      if (ia + rep * CPX_WAVE + lane < ib)
+        pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc & HARD_SPARSE, ia + rep * CPX_WAVE + lane);
     }
     uint32_t lw = 0;
     for (uint32_t k = 0; k < cend; k++) {
       if ((k & 31) == 0) lw = sh.u.st.lmask[k >> 5];
       const uint32_t p = sh.u.st.p[k];
       if ((lw >> (k & 31)) & 1u) {
-        // ---- leaves of p (state: crossed by p_1 .. p_{b-1}); needs every
-        //      thread's words and counts, so the block meets here
-        publish_counts<T, K>(cnt, cnt_lo, sh);
-        __syncthreads();
-        uint32_t wp[NW];
-        (void)read_wave_prefix<T, K>(sh, wp);
-        const uint32_t cc = sh.u.st.ccnt[k], cb = sh.u.st.cbase[k];
-        const bool sparse = cc >> 31;
-        const uint32_t ncand = cc & 0x7fffffffu;
-        const uint64_t xp = pf_xp;                            // x / p, fetched ahead
-        const double xpf = (double)xp;
-        int32_t smu = 0;
-        int32_t v = pf;
-        const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
-        for (uint32_t i = i0; i < ncand; i += step) {
-          if (i != i0) v = hard_fetch(bd, cb, sparse, i);
-          uint64_t m;
-          if (sparse) {
-            m = (uint32_t)v;                                  // prime, mu = -1
-            v = -1;
-          } else {
-            const uint32_t av = (uint32_t)(v < 0 ? -v : v);
-            if (av <= p) continue;                            // mu(m) = 0 or lpf(m) <= p
-            m = 2 * (uint64_t)(cb - i) + 1;
-          }
-          const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo <= n < hi (< 2^50)
-          const uint32_t c = count_below<T, K>(lds, sh, wp, (uint32_t)((n - lo + 1) >> 1));
-          if (v > 0) { acc += c; smu++; } else { acc -= c; smu--; }
-        }
-        pf_k = nk;
-        if (nk < cend) {
-          const uint32_t c2 = sh.u.st.ccnt[nk];
-          pf_xp = st[s0 + nk].xp;
-          if (i0 < (c2 & 0x7fffffffu)) pf = hard_fetch(bd, sh.u.st.cbase[nk], c2 >> 31, i0);
-        }
+        // ---- leaves of p whose n falls in this wave's bits (state: crossed
+        //      by p_1 .. p_{b-1}); the wave publishes its own prefix, no
+        //      other wave is involved
         {
-          const int32_t wm = (int32_t)wave_sum((uint32_t)smu);   // one LDS add per wave
-          if (wm && lane == 0) atomicAdd(&sh.u.st.msum[k], wm);
+          const uint32_t incl = wave_incl_scan(cnt);
+          sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
-        __syncthreads();                                      // readers done before cross-off
+        const uint32_t cc = sh.u.st.ccnt[k], cb = sh.u.st.cbase[k];
+        const bool sparse = cc & HARD_SPARSE;
+        const uint64_t xp = sh.u.st.xp[k];
+        const double xpf = (double)xp;
+        uint32_t ia, ib;
+        hard_wave_range<T, K>(bd, q, sh, k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
+        int32_t smu = 0;
+        if (ib > ia) {
+          int32_t v = pf;
+          for (uint32_t i = ia + rep * CPX_WAVE + lane; i < ib; i += cstep) {
+            if (i != ia + rep * CPX_WAVE + lane) v = hard_fetch(bd, cb, sparse, i);
+            uint64_t m;
+            int32_t sg = -1;
+            if (sparse) {
+              m = (uint32_t)v;                              // prime, mu = -1
+            } else {
+              const uint32_t av = (uint32_t)(v < 0 ? -v : v);
+              if (av <= p) continue;                        // mu(m) = 0 or lpf(m) <= p
+              m = 2 * (uint64_t)(cb - i) + 1;
+              sg = v > 0 ? 1 : -1;
+            }
+            const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo_w <= n < hi_w
+            const uint32_t c = count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
+            if (sg > 0) { acc += c; smu++; } else { acc -= c; smu--; }
+          }
+          // sum of mu over the wave (one candidate per lane at most: two ballots)
+          int32_t wm;
+          if (ib - ia <= cstep)
+            wm = (int32_t)__popcll(__ballot(smu > 0)) - (int32_t)__popcll(__ballot(smu < 0));
+          else
+            wm = (int32_t)wave_sum((uint32_t)smu);
+          if (lane == 0) sh.u.st.wm[k][wv] = wm;
+        }
+        const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
+        if (nk < cend) {
+          uint32_t na, nb;
+          hard_wave_range<T, K>(bd, q, sh, nk, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, na, nb);
+          if (na + rep * CPX_WAVE + lane < nb)
+            pf = hard_fetch(bd, sh.u.st.cbase[nk], sh.u.st.ccnt[nk] & HARD_SPARSE,
+                            na + rep * CPX_WAVE + lane);
+        }
       }
-      // ---- cross off the odd multiples of p in this thread's own bits
+      // ---- cross off the odd multiples of p in this thread's own bits,
+      //      from (t * 32K) mod p by the f32 reciprocal (t * 32K < 2^20, so
+      //      the truncated estimate is the quotient or one off either way)
       {
         const uint32_t oseg = sh.u.st.oseg[k];
-        const uint32_t tb = cpx_fastmod32(t * BPT, p, sh.u.st.magic[k]);
+        uint32_t tb = b0;
+        if (p <= b0) {                                    // p < 2^20 here
+          const uint32_t qe = (uint32_t)(fb0 * sh.u.st.rinv[k]);
+          int32_t r = (int32_t)(b0 - __umul24(qe, p));
+          if (r < 0) r += (int32_t)p;
+          else if (r >= (int32_t)p) r -= (int32_t)p;
+          tb = (uint32_t)r;
+        }
         uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
         const uint32_t a0 = t * (K + 1);
-        uint32_t d = 0, dlo = 0;
-        for (; j < BPT; j += p) {
-          const uint32_t addr = a0 + (j >> 5);
-          const uint32_t bit = 1u << (j & 31u);
-          const uint32_t w = lds[addr];
-          if (w & bit) { lds[addr] = w & ~bit; d++; dlo += j < BPT / 2; }
+        uint32_t wd;
+        if (p > BPT) {                                    // at most one multiple per thread
+          uint32_t hit = 0;
+          if (j < BPT) {
+            const uint32_t addr = a0 + (j >> 5);
+            const uint32_t bit = 1u << (j & 31u);
+            const uint32_t w = lds[addr];
+            if (w & bit) { lds[addr] = w & ~bit; hit = 1; }
+          }
+          cnt -= hit;
+          cnt_lo -= hit & (j < BPT / 2 ? 1u : 0u);
+          wd = (uint32_t)__popcll(__ballot(hit));
+        } else {
+          uint32_t d = 0, dlo = 0;
+          for (; j < BPT; j += p) {
+            const uint32_t addr = a0 + (j >> 5);
+            const uint32_t bit = 1u << (j & 31u);
+            const uint32_t w = lds[addr];
+            if (w & bit) { lds[addr] = w & ~bit; d++; dlo += j < BPT / 2; }
+          }
+          cnt -= d;
+          cnt_lo -= dlo;
+          wd = wave_sum(d);
         }
-        cnt -= d;
-        cnt_lo -= dlo;
-        const uint32_t wd = wave_sum(d);                       // one LDS add per wave
-        if (wd && lane == 0) atomicAdd(&sh.u.st.dec[k], wd);
+        if (lane == 0) sh.u.st.dw[k][wv] = (uint16_t)wd;   // this wave's removals
       }
     }
     nact = s0 + cend;
-    hard_flush<T, K>(sh, s0, nact, running, outT, outM, rep == 0);
+    hard_flush<T, K>(sh, s0, nact, running, outT, outM, rep == 0, acc);
     if (cend < clen || nact >= nst) break;
     s0 = nact;
     clen = min((uint32_t)HARD_SCH, nst - s0);
@@ -421,19 +607,41 @@ __device__ __forceinline__ void p2_segment(const BatchDev &bd, const QueryDev &q
   }
   for (uint32_t k = t; k < ncoop; k += T) {
     const uint32_t p = sp[i0 + k];
-    sh.u.offs[k] = (uint32_t)first_odd_multiple_bit(s0, p, (uint64_t)p * p);
+    sh.u.p2.offs[k] = (uint32_t)first_odd_multiple_bit_r(s0, p, (uint64_t)p * p);
+    sh.u.p2.p[k] = p;
+    sh.u.p2.rinv[k] = 1.0f / (float)p;
   }
   __syncthreads();
-  for (uint32_t k = 0; k < ncoop; k++) {
-    const uint32_t p = sp[i0 + k];
-    for (uint64_t j = sh.u.offs[k] + (uint64_t)t * p; j < SB; j += (uint64_t)T * p)
-      atomicAnd(&lds[seg_addr((uint32_t)(j >> 5), K, T)], ~(1u << (j & 31)));
+  // shared primes: every thread clears the multiples in its own words (no
+  // other thread writes them); its first one from (b0 - off) mod p by a float
+  // reciprocal (b0 - off < 2^21, so the estimate is within one either way)
+  {
+    const uint32_t b0 = t * BPT, a0 = t * (K + 1);
+    for (uint32_t k = 0; k < ncoop; k++) {
+      const uint32_t p = sh.u.p2.p[k], off = sh.u.p2.offs[k];
+      uint32_t j;
+      if (off >= b0) {
+        j = off - b0;
+      } else {
+        const uint32_t a = b0 - off;
+        const uint32_t qe = (uint32_t)((float)a * sh.u.p2.rinv[k]);
+        int32_t r = (int32_t)(a - qe * p);
+        if (r < 0) r += (int32_t)p;
+        else if (r >= (int32_t)p) r -= (int32_t)p;
+        j = r ? p - (uint32_t)r : 0u;
+      }
+      for (; j < BPT; j += p) atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31)));
+    }
   }
   for (uint32_t k = i0 + ncoop + t; k < np_all; k += T) {   // one large prime per thread
     const uint32_t p = sp[k];
     if ((uint64_t)p * p >= s1) break;
-    for (uint64_t j = first_odd_multiple_bit(s0, p, (uint64_t)p * p); j < SB; j += p)
-      atomicAnd(&lds[seg_addr((uint32_t)(j >> 5), K, T)], ~(1u << (j & 31)));
+    const uint64_t o = first_odd_multiple_bit_r(s0, p, (uint64_t)p * p);
+    if (o >= SB) continue;
+    for (uint32_t j = (uint32_t)o; j < (uint32_t)SB; j += p) {   // SB + p < 2^32
+      const uint32_t g = j >> 5;
+      atomicAnd(&lds[g + g / K], ~(1u << (j & 31)));
+    }
   }
   __syncthreads();
   const uint32_t cnt_lo = own_count_lo<K>(lds, t);
@@ -513,7 +721,8 @@ size_t seg_lds_bytes(SegShape shape) { return (size_t)(shape.kwords + 1) * shape
 // One-wavefront workgroups (T = 64) make every block barrier a single-wave
 // barrier and let ~15 independent segment workers share a CU.
 #define CPX_SEG_SHAPES(X) \
-  X(64, 32) X(64, 64) X(64, 128) X(128, 32) X(128, 64) X(256, 32) X(256, 64) X(512, 32)
+  X(64, 32) X(64, 64) X(64, 128) X(128, 32) X(128, 64) X(256, 32) X(256, 64) X(256, 128) \
+  X(512, 32) X(512, 64)
 
 bool seg_shape_supported(SegShape s)
 {
