@@ -82,6 +82,21 @@ def algorithmic_lds_bytes(queries, primes):
     return tot
 
 
+def profiled_traffic(workload):
+    """HBM bytes per k_segments launch from the committed rocprofv3 --pmc
+    summary of the same workload (FETCH_SIZE and WRITE_SIZE, scaled by the
+    copy-kernel calibration recorded there), or None."""
+    path = os.path.join(ROOT, "profiles", "k_segments_%s_pmc.md" % workload)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(xs, budget_s=20.0):
     """Oracle (C + OpenMP) on a stratified sample of the workload."""
     from oracle import oracle as O
@@ -299,13 +314,27 @@ def main():
         # each served result equals the same x solved alone
         out["served"] = served
 
-    # ---- roofline of the persistent segment kernel
+    # ---- roofline of the persistent segment kernel (the LDS bound of the
+    # staged sieve; HBM and the integer VALU rate beside it)
+    lds_peak, int_peak = ctx.probe_peaks()
     ach = w_lds / seg_avg_s / 1e9
-    out["roofline"] = {"bound": "lds", "achieved": round(ach, 1), "peak": None, "unit": "GB/s",
-                       "frac": None, "traffic": None,
+    traffic = profiled_traffic("batch")
+    out["roofline"] = {"bound": "lds", "achieved": round(ach, 1), "peak": round(lds_peak / 1e9, 1),
+                       "unit": "GB/s", "frac": round(ach / (lds_peak / 1e9), 4),
+                       "traffic": traffic,
                        "kernel": "k_segments (persistent hard-leaf + P2 tiles)",
                        "kernel_ms": round(seg_avg_s * 1000, 3),
-                       "algorithmic_bytes": w_lds}
+                       "algorithmic_bytes": w_lds,
+                       "peak_source": "measured on this device by cpx_probe_peaks (LDS 16-byte reads, all CUs)",
+                       "components": {
+                           "hbm": {"achieved": round(traffic / seg_avg_s / 1e9, 1) if traffic else None,
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(traffic / seg_avg_s / 1e9 / HBM_PEAK_GBS, 5) if traffic else None},
+                           "int32_valu": {"peak": round(int_peak / 1e12, 2), "unit": "Tops/s (lane ops)"}}}
+    if "single_1e16" in out:
+        s16 = out["single_1e16"]
+        s16["lds_roofline_frac"] = round(s16["lds_algorithmic_GB"] / (s16["segments_ms"] / 1000.0)
+                                         / (lds_peak / 1e9), 4)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(xs)

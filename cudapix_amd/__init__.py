@@ -91,7 +91,7 @@ EXPORTS = [
     "cpx_pi_batch", "cpx_pi_batch_terms", "cpx_autotune",
     "cpx_server_create", "cpx_server_submit", "cpx_server_cancel", "cpx_server_poll",
     "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
-    "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave",
+    "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave", "cpx_probe_peaks",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -173,6 +173,7 @@ def lib():
         "cpx_request_bytes": (U64, [vp, CpxU128]),
         "cpx_selftest_arith": (ctypes.c_int, [U64, U64]),
         "cpx_selftest_wave": (ctypes.c_int, [vp, U64, ctypes.c_uint32, P(ctypes.c_uint32)]),
+        "cpx_probe_peaks": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -385,6 +386,12 @@ class Context:
 
     def device_bytes(self):
         return self._L.cpx_ctx_device_bytes(self.h)
+
+    def probe_peaks(self):
+        """Measured LDS read bandwidth (bytes/s) and int32 VALU rate (lane ops/s)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._L.cpx_probe_peaks(self.h, ctypes.byref(a), ctypes.byref(b)), "probe_peaks")
+        return a.value, b.value
 
     def selftest_wave(self, seed=1, nwaves=4096):
         """Wavefronts whose register scan/sum differ from a serial scan."""

@@ -755,3 +755,47 @@ int cpx_selftest_wave(cpx_ctx *c, uint64_t seed, uint32_t nwaves, uint32_t *bad)
   (void)hipFree(d);
   return rc;
 }
+
+// ------------------------------------------------------------ peak probes
+hipError_t probe_lds(uint32_t blocks, uint32_t iters, uint32_t *sink, hipStream_t s);
+hipError_t probe_valu(uint32_t blocks, uint32_t iters, uint32_t *sink, hipStream_t s);
+
+int cpx_probe_peaks(cpx_ctx *c, double *lds_bytes_per_s, double *int32_ops_per_s)
+{
+  CTX_GUARD(c);
+  if (!lds_bytes_per_s || !int32_ops_per_s) return CPX_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const uint32_t blocks = 8u * (uint32_t)std::max(c->cu_count, 1);
+  const uint32_t lds_iters = 4096, valu_iters = 16384;
+  uint32_t *sink = nullptr;
+  HIPCHK(hipMalloc(&sink, blocks * sizeof(uint32_t)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = CPX_OK;
+  do {
+    hipError_t e = hipEventCreate(&e0);
+    if (!e) e = hipEventCreate(&e1);
+    float ms = 0;
+    // LDS: 8 x 16-byte reads per thread per iteration (plus one warm-up launch)
+    if (!e) e = probe_lds(blocks, lds_iters, sink, c->stream);
+    if (!e) e = hipEventRecord(e0, c->stream);
+    if (!e) e = probe_lds(blocks, lds_iters, sink, c->stream);
+    if (!e) e = hipEventRecord(e1, c->stream);
+    if (!e) e = hipEventSynchronize(e1);
+    if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e) { rc = hip_fail(c, e, "probe_lds"); break; }
+    *lds_bytes_per_s = (double)blocks * 256.0 * lds_iters * 8.0 * 16.0 / (ms * 1e-3);
+    // VALU: 24 int32 instructions per lane per iteration (8 x add, shift, xor)
+    if (!e) e = probe_valu(blocks, valu_iters, sink, c->stream);
+    if (!e) e = hipEventRecord(e0, c->stream);
+    if (!e) e = probe_valu(blocks, valu_iters, sink, c->stream);
+    if (!e) e = hipEventRecord(e1, c->stream);
+    if (!e) e = hipEventSynchronize(e1);
+    if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e) { rc = hip_fail(c, e, "probe_valu"); break; }
+    *int32_ops_per_s = (double)blocks * 256.0 * valu_iters * 24.0 / (ms * 1e-3);
+  } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(sink);
+  return rc;
+}

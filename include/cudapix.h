@@ -225,6 +225,12 @@ int cpx_selftest_arith(uint64_t seed, uint64_t n);
  * of wavefronts that differ. */
 int cpx_selftest_wave(cpx_ctx *ctx, uint64_t seed, uint32_t nwaves, uint32_t *bad);
 
+/* ---- measured peaks (no reference counterpart) ---------------------------- */
+/* LDS read bandwidth (bytes/s, conflict-free 16-byte reads on every CU) and
+ * 32-bit integer VALU rate (lane operations/s) of the context's device, timed
+ * with HIP events: the denominators of the roofline fractions. */
+int cpx_probe_peaks(cpx_ctx *ctx, double *lds_bytes_per_s, double *int32_ops_per_s);
+
 #ifdef __cplusplus
 }
 #endif
