@@ -54,7 +54,8 @@ struct QueryDev {
   uint64_t hard_nseg, hard_row0;      // hard segments and first output row (T, bact, rowinfo)
   uint64_t m_off;                     // first element of this query in hard_M
   uint64_t mrow_base, hard_nmrow;     // first M-row (acc slot) and M-row count (>= segments)
-  uint64_t p2_nseg, p2_row0;          // P2 segments and first output row
+  uint64_t p2_nseg, p2_row0;          // segments holding P2 leaves and their first output row
+  uint64_t p2_seg0;                   // first of those segments (they run to the one holding zt)
   uint64_t stage_off;                 // first entry of this query in the stage table
   uint64_t part_off;                  // first slot in the partial-sum array
   uint32_t n_ord_parts, n_triv_parts, n_easy_parts, n_hard_parts;
@@ -93,7 +94,7 @@ __host__ __device__ __forceinline__ uint32_t cpx_fastmod32(uint32_t a, uint32_t 
 // Work tile of the persistent segment kernel.  A leaf-heavy hard segment is
 // split into nrep replica tiles: each sieves the whole segment but evaluates
 // only the leaves with index = rep (mod nrep), writing its own sum(mu) row.
-enum { TILE_HARD = 0, TILE_P2 = 1 };
+enum { TILE_HARD = 0 };
 struct TileDev {
   uint32_t query;
   uint16_t kind;

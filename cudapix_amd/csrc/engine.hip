@@ -341,9 +341,12 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   d.pi_cbrt = (uint32_t)host_pi(c, std::min(h.cbrtx, h.y));
   d.pi_qrt = (uint32_t)B;
   d.pi_sqrtz = (uint32_t)host_pi(c, h.sqrtz);
-  // hard segments over [0, z]; P2 segments over [p2_e, zt]
-  d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : 0;
-  d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? ceil_div(h.zt + 1 - d.p2_e, 2 * SB) : 0;
+  // segments of 2*SB integers from 0: the hard leaves use [0, z]; P2 counts
+  // primes in (p2_e, zt] in the segments that hold them (a P2 epilogue of
+  // the same tile), so the tiles reach z, or zt when only P2 is asked for
+  d.p2_seg0 = d.p2_e / (2 * SB);
+  d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? h.zt / (2 * SB) + 1 - d.p2_seg0 : 0;
+  d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : (d.p2_nseg ? d.p2_seg0 + d.p2_nseg : 0);
   uint32_t part = 0;
   // ordinary: odd n <= y
   if (mask & TERM_ORD) {
@@ -396,6 +399,7 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   const double leaf_target = 256.0 * T;                // leaf evaluations per replica
   uint32_t sparse_run = 0;
   uint64_t mrows = 0, toff = 0, moff = 0;
+  uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
   uint64_t act = d.nstage;                             // non-increasing in the segment
   qp.rowinfo.reserve(d.hard_nseg);
   qp.tiles.reserve(d.hard_nseg + d.p2_nseg);
@@ -434,24 +438,23 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
     moff += (uint64_t)nrep * act;
     const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
                         2.0 * leaves / ((double)nrep * T);
-    const uint16_t bk = cost_bucket(cost);
+    // replica 0 of a segment of the P2 range also crosses the primes from
+    // p_(c+act+1) to sqrt(hi) and counts the P2 leaves there
+    double p2cost = 0;
+    if (s >= d.p2_seg0 && s < d.p2_seg0 + d.p2_nseg) {
+      const double sq = std::min(std::sqrt((double)hi), (double)c->prime_limit);
+      while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
+      const double nsp = (double)nsp_i - (double)(h.c + act);
+      p2cost = 2.0 + std::max(nsp, 0.0) * 8.0 / T;
+    }
     for (uint32_t r = 0; r < nrep; r++) {
       qp.tiles.push_back({i, (uint16_t)TILE_HARD, (uint16_t)r, s});
-      qp.tbucket.push_back(bk);
+      qp.tbucket.push_back(cost_bucket(r == 0 ? cost + p2cost : cost));
     }
   }
   d.hard_nmrow = mrows;
   qp.tm = toff;
   qp.mm = moff;
-  uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
-  for (uint64_t s = 0; s < d.p2_nseg; s++) {
-    const double sq = std::min(std::sqrt((double)(d.p2_e + (s + 1) * 2 * SB)), (double)c->prime_limit);
-    while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
-    const double nsp = (double)nsp_i;
-    double cost = 2.0 + nsp * 8.0 / T + (double)SB * 3.0 / (64.0 * T);
-    qp.tiles.push_back({i, (uint16_t)TILE_P2, (uint16_t)0, s});
-    qp.tbucket.push_back(cost_bucket(cost));
-  }
 }
 
 // Run f(k) for k in [0, n) on up to `threads` host threads (inline for one).

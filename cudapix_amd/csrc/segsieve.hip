@@ -1,41 +1,38 @@
-// segsieve.hip -- the persistent segment kernel: hard-leaf phi sieve and the
-// P2 segmented prime count, one LDS-resident bitmap segment per tile.
+// segsieve.hip -- the persistent segment kernel: hard-leaf phi sieve of one
+// LDS-resident bitmap segment per tile, with the P2 prime count of the same
+// segment as an epilogue.
 //
-// Layout: a segment holds 32*K*T odd numbers lo+1, lo+3, ... as bits
-// (1 = unsieved) in padded-linear LDS; thread t owns the contiguous bits
-// [t*32K, (t+1)*32K) and keeps their survivor count in a register.
+// Layout: a segment of [lo, lo + 2*SB) holds its SB = 32*K*T odd numbers
+// lo+1, lo+3, ... as bits (1 = unsieved) in padded-linear LDS; thread t owns
+// the contiguous bits [t*32K, (t+1)*32K) and keeps their survivor count (and
+// that of its first K/2 words) in registers; wave w owns 64 threads' bits.
 //
 // Hard leaves (replaces S2glob::S2ctl + S2hardHost, src/sieve/S2_hard_device.cu:264-355,
 // src/sieve/S2_hard_host.cu:115-191; DR split p <= x^(1/4), src/Deleglise-Rivat/S3.cu:8):
-//   The segment starts as the wheel of the first c primes.  For each stage b
-//   (p = p_b, c < b <= B, while p^2 <= x/lo):
-//     L  leaves of p: the whole block strides the segment's m range
-//        (x/(p*hi), x/(p*lo)] -- odd m with mu(m) != 0 and lpf(m) > p read
-//        coalesced from the mu*lpf table when p^2 <= y, primes m from the
-//        prime list when p^2 > y.  phi_local(n) for n = x/(m p) is
-//          wave prefix (registers) + in-wave prefix of the owning thread
-//          (LDS) + popcount of the owner's words up to n;
-//        sum mu * phi_local and sum mu accumulate per thread.
-//     -- barrier --
-//     X  cross off the odd multiples of p in the thread's own bits (private
-//        words, no atomics), update its count, wave-scan the counts and
-//        publish in-wave prefixes and wave totals for the next stage.
-//     -- barrier --
-//   Per segment it writes T_b (survivors before stage b), sum_b(mu) and the
-//   local sum; hard_finalize adds the carry sum_b prefix_b(seg) * sum_b(mu)
-//   (the decomposition the reference does with scanVectorized/addMultiply,
+//   The segment starts as the wheel of the first c primes.  Stages b (p = p_b,
+//   c < b <= B, while p^2 <= x/lo) run in chunks of HARD_SCH: the block first
+//   tabulates the chunk (p, x/p, first multiple, leaf candidate range of the
+//   segment and its split among the waves), then every wave walks the chunk's
+//   stages on its own, with no block barrier:
+//     L  leaves of p whose n = x/(p m) falls in the wave's bits: m odd with
+//        mu(m) != 0 and lpf(m) > p from the mu*lpf table (p^2 <= y) or m prime
+//        (p^2 > y); the wave publishes its in-wave prefix counts and counts
+//        phi_wave(n) = in-wave prefix of the owning lane + popcounts of its
+//        words below n; sum mu * phi_wave per thread, sum mu per wave;
+//     X  cross off the odd multiples of p in the thread's own bits, recording
+//        each wave's removals.
+//   At the end of a chunk the block adds the survivors of the earlier waves
+//   to every leaf's count (from the per-stage, per-wave removals and sums of
+//   mu), and writes T_b (survivors before stage b) and sum_b(mu); the carry
+//   kernel adds sum_b prefix_b(seg) * sum_b(mu) over the segments (the
+//   decomposition the reference does with scanVectorized/addMultiply,
 //   src/sieve/S2_hard_device.cu:357-375).
 //
-// P2 (replaces the windowed upper_bound count of src/P2.cu:76-133):
-//   segments of [sqrt(x) rounded down to 64, x/(y+1)] are fully sieved (wheel,
-//   then the small primes shared by the block from offsets computed once per
-//   segment, then one large prime per thread), then the block strides the P2
-//   primes p (pi-table bits) whose x/p lands in the segment and counts primes
-//   <= x/p the same way.  Per segment: prime count, number of p, sum of local
-//   counts; the carry is applied in the combine kernel.
+// P2: the segments holding x/p for the P2 primes p in (y, sqrt x] get, on
+// replica 0 of their tile, the rest of a prime sieve (primes above the last
+// stage up to sqrt(hi)) and the P2 counts; see p2_epilogue.
 #include "cpx_device.hpp"
 
-#define P2_MAX_COOP 512    // LDS slots for shared-prime offsets
 #define HARD_SCH 128       // stages per chunk of the hard stage table
 
 template <int T, int K>
@@ -46,11 +43,6 @@ struct SegShared {
   unsigned long long red_u[T / CPX_WAVE];
   long long red_s[T / CPX_WAVE];
   union {
-    struct {                     // P2: the block-shared (small) sieving primes
-      uint32_t offs[P2_MAX_COOP];  // first odd multiple >= max(s0 + 1, p^2), as a bit
-      uint32_t p[P2_MAX_COOP];
-      float rinv[P2_MAX_COOP];     // 1 / p
-    } p2;
     struct {                     // hard: stage table and per-wave counters of the chunk
       uint64_t xp[HARD_SCH];     // floor(x / p)
       uint32_t p[HARD_SCH];
@@ -387,6 +379,99 @@ __device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uin
   __syncthreads();
 }
 
+// P2 leaves of a segment (replaces the windowed upper_bound count of
+// src/P2.cu:76-133), run by replica 0 of a hard tile after its stages: the
+// bitmap already holds [lo, hi) sieved by the wheel and the stage primes
+// p_(c+1) .. p_kfirst; the primes from index kfirst with p^2 < hi are
+// crossed here (one per thread, from p^2), which leaves exactly the primes
+// above p2_e.  Then for each P2 prime p in (max(y, x/hi), min(sqrt x, x/lo)]
+// (pi-table bits, block-strided) the count of primes in (p2_e, x/p] held in
+// this segment; per segment: primes in (p2_e, hi), number of p, sum of the
+// counts.  The carry is applied in the combine kernel.
+template <int T, int K>
+__device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const QueryDev &q, uint64_t seg,
+                                            uint64_t lo, uint32_t kfirst, uint32_t *lds,
+                                            SegShared<T, K> &sh)
+{
+  constexpr uint32_t NW = T / CPX_WAVE;
+  constexpr uint32_t BPT = 32 * K;
+  const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
+  const uint64_t SB = (uint64_t)BPT * T;
+  const uint64_t hi = lo + 2 * SB;
+  const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+  const uint32_t *__restrict__ sp = bd.primes;
+  const uint32_t np_all = (uint32_t)bd.nprimes;
+
+  __syncthreads();
+  for (uint32_t k = kfirst + t; k < np_all; k += T) {
+    const uint32_t p = sp[k];
+    if ((uint64_t)p * p >= hi) break;
+    const uint64_t o = first_odd_multiple_bit_r(lo, p, (uint64_t)p * p);
+    if (o >= SB) continue;
+    for (uint32_t j = (uint32_t)o; j < (uint32_t)SB; j += p) {   // SB + p < 2^32
+      const uint32_t g = j >> 5;
+      atomicAnd(&lds[g + g / K], ~(1u << (j & 31)));
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt_lo = own_count_lo<K>(lds, t);
+  uint32_t cnt = cnt_lo;
+#pragma unroll 8
+  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
+  publish_counts<T, K>(cnt, cnt_lo, sh);
+  __syncthreads();
+  uint32_t wp[NW];
+  const uint32_t tot = read_wave_prefix<T, K>(sh, wp);
+  // survivors at or below p2_e (1, the stage primes and what is below
+  // sqrt x are not counted): only the first segment of the range has any
+  const uint32_t ce = q.p2_e >= lo ? count_below<T, K>(lds, sh, wp, (uint32_t)min((q.p2_e - lo + 1) >> 1, SB)) : 0u;
+
+  unsigned long long loc = 0;
+  uint32_t np = 0;
+  uint64_t pmax = q.sqrtx;
+  if (lo) {
+    const uint64_t v = cpx_div128_to64(x, lo);
+    if (v < pmax) pmax = v;
+  }
+  uint64_t pmin = cpx_div128_to64(x, hi);
+  if (pmin < q.y) pmin = q.y;
+  if (pmax > pmin) {
+    const uint64_t e_lo = (pmin + 1) >> 6, e_hi = pmax >> 6;
+    for (uint64_t e = e_lo + t; e <= e_hi; e += T) {
+      const PiEntry pe = bd.pitab[e];
+      uint32_t b = pe.bits;
+      const uint64_t base = 64 * e;
+      // keep odd integers base + 2i + 1 in (pmin, pmax]
+      if (pmax < base + 63) b &= cpx_lowmask32((uint32_t)((pmax - base + 1) >> 1));
+      if (pmin > base) {
+        const uint64_t kk = (pmin - base + 1) >> 1;     // bits for integers <= pmin
+        b = kk >= 32 ? 0u : (b & ~cpx_lowmask32((uint32_t)kk));
+      }
+      while (b) {
+        const uint32_t i = __builtin_ctz(b);
+        b &= b - 1;
+        const uint64_t p = base + 2 * i + 1;
+        const uint64_t v = cpx_div128_to64(x, p);       // lo <= v < hi
+        loc += (unsigned long long)(count_below<T, K>(lds, sh, wp, (uint32_t)((v - lo + 1) >> 1)) - ce);
+        np++;
+      }
+    }
+  }
+  loc = wave_sum(loc);
+  const uint32_t npw = wave_sum(np);
+  if (lane == 0) { sh.red_u[wv] = loc; sh.wsum[wv] = (int32_t)npw; }
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long L = 0;
+    uint32_t N = 0;
+    for (uint32_t w = 0; w < NW; w++) { L += sh.red_u[w]; N += (uint32_t)sh.wsum[w]; }
+    const uint64_t row = q.p2_row0 + (seg - q.p2_seg0);
+    bd.p2_cnt[row] = tot - ce;
+    bd.p2_np[row] = N;
+    bd.p2_loc[row] = L;
+  }
+}
+
 template <int T, int K>
 __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev &q,
                                              const TileDev &tile, uint32_t *lds, SegShared<T, K> &sh)
@@ -574,131 +659,13 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
     bd.hard_acc[q.mrow_base + mrow] = tt;
     if (rep == 0) bd.hard_bact[row] = nact;
   }
+  if (rep == 0 && seg >= q.p2_seg0 && seg < q.p2_seg0 + q.p2_nseg)
+    p2_epilogue<T, K>(bd, q, seg, lo, q.c + nact, lds, sh);
 }
 
 template <int T, int K>
-__device__ __forceinline__ void p2_segment(const BatchDev &bd, const QueryDev &q, uint64_t seg,
-                                           uint32_t *lds, SegShared<T, K> &sh)
-{
-  constexpr uint32_t NW = T / CPX_WAVE;
-  constexpr uint32_t BPT = 32 * K;
-  const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
-  const uint64_t SB = (uint64_t)BPT * T;
-  const uint64_t s0 = q.p2_e + seg * 2 * SB;
-  const uint64_t s1 = s0 + 2 * SB;
-  const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
-  const WheelInfo wi = bd.winfo[q.c];
-  const uint32_t *__restrict__ sp = bd.primes;
-  const uint32_t np_all = (uint32_t)bd.nprimes;
-
-  // wheel + offsets of the block-shared primes (computed once, in parallel)
-  seg_fill_wheel(lds, K, T, s0, bd.wheel + wi.off, wi.period);
-  const uint32_t i0 = q.c;
-  uint32_t ncoop = 0;                                    // shared primes: p < BPT
-  {
-    // primes[i0 + k] < BPT for k < ncoop; bounded by the LDS slots
-    uint32_t lo_i = i0, hi_i = min(np_all, i0 + (uint32_t)P2_MAX_COOP);
-    while (lo_i < hi_i) {                                // first index with p >= BPT or p^2 >= s1
-      uint32_t mid = (lo_i + hi_i) / 2;
-      uint64_t pm = sp[mid];
-      if (pm < BPT && pm * pm < s1) lo_i = mid + 1; else hi_i = mid;
-    }
-    ncoop = lo_i - i0;
-  }
-  for (uint32_t k = t; k < ncoop; k += T) {
-    const uint32_t p = sp[i0 + k];
-    sh.u.p2.offs[k] = (uint32_t)first_odd_multiple_bit_r(s0, p, (uint64_t)p * p);
-    sh.u.p2.p[k] = p;
-    sh.u.p2.rinv[k] = 1.0f / (float)p;
-  }
-  __syncthreads();
-  // shared primes: every thread clears the multiples in its own words (no
-  // other thread writes them); its first one from (b0 - off) mod p by a float
-  // reciprocal (b0 - off < 2^21, so the estimate is within one either way)
-  {
-    const uint32_t b0 = t * BPT, a0 = t * (K + 1);
-    for (uint32_t k = 0; k < ncoop; k++) {
-      const uint32_t p = sh.u.p2.p[k], off = sh.u.p2.offs[k];
-      uint32_t j;
-      if (off >= b0) {
-        j = off - b0;
-      } else {
-        const uint32_t a = b0 - off;
-        const uint32_t qe = (uint32_t)((float)a * sh.u.p2.rinv[k]);
-        int32_t r = (int32_t)(a - qe * p);
-        if (r < 0) r += (int32_t)p;
-        else if (r >= (int32_t)p) r -= (int32_t)p;
-        j = r ? p - (uint32_t)r : 0u;
-      }
-      for (; j < BPT; j += p) atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31)));
-    }
-  }
-  for (uint32_t k = i0 + ncoop + t; k < np_all; k += T) {   // one large prime per thread
-    const uint32_t p = sp[k];
-    if ((uint64_t)p * p >= s1) break;
-    const uint64_t o = first_odd_multiple_bit_r(s0, p, (uint64_t)p * p);
-    if (o >= SB) continue;
-    for (uint32_t j = (uint32_t)o; j < (uint32_t)SB; j += p) {   // SB + p < 2^32
-      const uint32_t g = j >> 5;
-      atomicAnd(&lds[g + g / K], ~(1u << (j & 31)));
-    }
-  }
-  __syncthreads();
-  const uint32_t cnt_lo = own_count_lo<K>(lds, t);
-  uint32_t cnt = cnt_lo;
-#pragma unroll 8
-  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
-  publish_counts<T, K>(cnt, cnt_lo, sh);
-  __syncthreads();
-  uint32_t wp[NW];
-  const uint32_t tot = read_wave_prefix<T, K>(sh, wp);
-
-  // P2 primes p in (max(y, x/s1), min(sqrt x, x/s0)]: block strides the pi-table entries
-  unsigned long long loc = 0;
-  uint32_t np = 0;
-  uint64_t pmax = cpx_div128_to64(x, s0);
-  if (pmax > q.sqrtx) pmax = q.sqrtx;
-  uint64_t pmin = cpx_div128_to64(x, s1);
-  if (pmin < q.y) pmin = q.y;
-  if (pmax > pmin) {
-    const uint64_t e_lo = (pmin + 1) >> 6, e_hi = pmax >> 6;
-    for (uint64_t e = e_lo + t; e <= e_hi; e += T) {
-      const PiEntry pe = bd.pitab[e];
-      uint32_t b = pe.bits;
-      const uint64_t base = 64 * e;
-      // keep odd integers base + 2i + 1 in (pmin, pmax]
-      if (pmax < base + 63) b &= cpx_lowmask32((uint32_t)((pmax - base + 1) >> 1));
-      if (pmin > base) {
-        const uint64_t kk = (pmin - base + 1) >> 1;     // bits for integers <= pmin
-        b = kk >= 32 ? 0u : (b & ~cpx_lowmask32((uint32_t)kk));
-      }
-      while (b) {
-        const uint32_t i = __builtin_ctz(b);
-        b &= b - 1;
-        const uint64_t p = base + 2 * i + 1;
-        const uint64_t v = cpx_div128_to64(x, p);       // s0 <= v < s1
-        loc += (unsigned long long)count_below<T, K>(lds, sh, wp, (uint32_t)((v - s0 + 1) >> 1));
-        np++;
-      }
-    }
-  }
-  loc = wave_sum(loc);
-  const uint32_t npw = wave_sum(np);
-  if (lane == 0) { sh.red_u[wv] = loc; sh.wsum[wv] = (int32_t)npw; }
-  __syncthreads();
-  if (t == 0) {
-    unsigned long long L = 0;
-    uint32_t N = 0;
-    for (uint32_t w = 0; w < NW; w++) { L += sh.red_u[w]; N += (uint32_t)sh.wsum[w]; }
-    const uint64_t row = q.p2_row0 + seg;
-    bd.p2_cnt[row] = tot;
-    bd.p2_np[row] = N;
-    bd.p2_loc[row] = L;
-  }
-}
-
-template <int T, int K>
-__global__ __launch_bounds__(T) void k_segments(BatchDev bd)
+// (at least 4 waves per SIMD: two 512-thread blocks per CU need <= 128 VGPRs)
+__global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
 {
   extern __shared__ uint32_t lds[];
   __shared__ SegShared<T, K> sh;
@@ -710,8 +677,7 @@ __global__ __launch_bounds__(T) void k_segments(BatchDev bd)
     if (ti >= ntiles) break;
     const TileDev tile = bd.tiles[ti];
     const QueryDev q = bd.queries[tile.query];
-    if (tile.kind == TILE_HARD) hard_segment<T, K>(bd, q, tile, lds, sh);
-    else p2_segment<T, K>(bd, q, tile.seg, lds, sh);
+    hard_segment<T, K>(bd, q, tile, lds, sh);
     __syncthreads();
   }
 }
