@@ -37,3 +37,16 @@ def test_pi_1e17(ctx):
 def test_pi_2_64_u128(ctx):
     # 2^64 needs the 128-bit path (x does not fit 64 bits)
     assert ctx.pi(2 ** 64) == KNOWN["pi2"]["64"]
+
+
+@pytest.mark.slow
+def test_pi_1e20_two_alphas(ctx):
+    # 128-bit x with 64-bit primes and u128 leaf quotients; the same total for
+    # alpha = 2 and 4 (and the built-in alpha), each the published pi(10^20)
+    x = 10 ** 20
+    cb = 4641588                                   # icbrt(10^20)
+    for alpha in (2.0, 4.0, 0.0):
+        y = int(alpha * cb)
+        print("case pi(1e20) alpha=%g y=%d" % (alpha, y), flush=True)
+        t = ctx.pi(x, y=y, terms=True)
+        assert t["pi"] == KNOWN["pi10"]["20"], (alpha, t)
