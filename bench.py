@@ -206,7 +206,8 @@ def main():
     log("warmup: autotune + %d batched steps" % args.warmup)
     t0 = time.time()
     shape = ctx.autotune(xs)
-    log("autotuned shape threads=%d kwords=%d in %.1fs" % (shape[0], shape[1], time.time() - t0))
+    tuning = ctx.tuning()
+    log("autotuned %s in %.1fs" % (tuning, time.time() - t0))
     ref = None
     for i in range(args.warmup):
         r = ctx.pi_batch(xs, use_graph=True)
@@ -245,6 +246,7 @@ def main():
            "config": {"workload": "batched pi(x), 512 queries x in [1e9,1e15], one persistent "
                                   "segment launch per step",
                       "queries": n, "seg_threads": shape[0], "seg_kwords": shape[1],
+                      "alpha_scale": tuning["alpha_scale"], "wheel_c": tuning["c"],
                       "parallelism": "single GPU"}}
 
     # ---- plain one-at-a-time solves (fresh tables per query) and warm-table variant
@@ -272,14 +274,17 @@ def main():
 
     # ---- single large query
     if not args.no_single:
+        # a context with the default (single-query) tuning
+        sc = C.Context(device=0)
         x16 = 10 ** 16
-        ctx.pi(x16)
+        sc.pi(x16)
         t0 = time.time()
-        t = ctx.pi(x16, terms=True)
+        t = sc.pi(x16, terms=True)
         d16 = time.time() - t0
         if t["pi"] != 279238341033925:
             raise SystemExit("pi(1e16) wrong: %d" % t["pi"])
-        _, st16 = ctx.pi_batch([x16], stats=True)
+        _, st16 = sc.pi_batch([x16], stats=True)
+        sc.close()
         t["x"] = x16
         w16 = algorithmic_lds_bytes([t], primes)
         out["single_1e16"] = {"ms": round(d16 * 1000, 2), "y": t["y"], "alpha": round(t["alpha"], 3),

@@ -92,6 +92,7 @@ EXPORTS = [
     "cpx_server_create", "cpx_server_submit", "cpx_server_cancel", "cpx_server_poll",
     "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
     "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave", "cpx_probe_peaks",
+    "cpx_get_tuning",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -174,6 +175,8 @@ def lib():
         "cpx_selftest_arith": (ctypes.c_int, [U64, U64]),
         "cpx_selftest_wave": (ctypes.c_int, [vp, U64, ctypes.c_uint32, P(ctypes.c_uint32)]),
         "cpx_probe_peaks": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
+        "cpx_get_tuning": (ctypes.c_int, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32),
+                                          P(ctypes.c_double), P(ctypes.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -380,6 +383,14 @@ class Context:
         self._chk(self._L.cpx_autotune(self.h, arr, n, ctypes.byref(t), ctypes.byref(k)),
                   "autotune")
         return t.value, k.value
+
+    def tuning(self):
+        """(threads, kwords, alpha_scale, c) the context currently uses."""
+        t, k, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        a = ctypes.c_double()
+        self._chk(self._L.cpx_get_tuning(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(a),
+                                         ctypes.byref(c)), "get_tuning")
+        return {"seg_threads": t.value, "seg_kwords": k.value, "alpha_scale": a.value, "c": c.value}
 
     def request_bytes(self, x):
         return self._L.cpx_request_bytes(self.h, u128(x))

@@ -648,41 +648,88 @@ int cpx_pi_batch_terms(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out, 
 // ------------------------------------------------------------ autotuner
 // Times every supported segment shape on the calibration batch (after one
 // warm-up) and keeps the fastest; results must agree bit for bit.
+// Times one calibration batch per candidate and keeps the fastest, in three
+// passes: the segment shape, then a factor on the alpha model, then the
+// number of wheel primes c.  Every candidate must return the same pi(x) for
+// every query (bit for bit), or the tuner fails.
 int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, uint32_t *kwords)
 {
   CTX_GUARD(c);
   if (!xs || !n) return CPX_EINVAL;
   static const SegShape cands[] = {{64, 32}, {64, 64}, {64, 128}, {128, 32}, {128, 64}, {256, 32},
                                    {256, 64}, {256, 128}, {512, 32}, {512, 64}};
+  static const double scales[] = {0.7, 0.85, 1.0, 1.2, 1.45};
+  static const uint32_t wheels[] = {6, 7, 8};
   std::vector<HostQuery> hq(n);
-  for (size_t i = 0; i < n; i++) {
-    int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);
-    if (rc) return rc;
-  }
-  SegShape best = c->shape;
-  double best_ms = 1e300;
+  auto make = [&]() -> int {
+    for (size_t i = 0; i < n; i++) {
+      int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);
+      if (rc) return rc;
+    }
+    return CPX_OK;
+  };
   std::vector<ResultDev> ref, res;
   bool have_ref = false;
-  for (const SegShape &s : cands) {
-    if (!seg_shape_supported(s) || seg_blocks_per_cu(s) < 1) continue;
-    c->shape = s;
-    int rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);   // warm-up (tables, arena)
+  // time one configuration (after a warm-up run for tables and arena)
+  auto trial = [&](double &ms) -> int {
+    int rc = make();
+    if (!rc) rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);
     if (rc) return rc;
-    double t0 = now_ms();
-    rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);
-    if (rc) return rc;
-    double ms = now_ms() - t0;
+    const double t0 = now_ms();
+    if ((rc = run_queries(c, hq, TERM_ALL, res, nullptr, false))) return rc;
+    ms = now_ms() - t0;
     if (!have_ref) { ref = res; have_ref = true; }
     for (size_t i = 0; i < n; i++)
       if (memcmp(res[i].pi, ref[i].pi, 16) != 0)
-        return ctx_fail(c, CPX_EDEVICE, "autotune: shapes disagree");
+        return ctx_fail(c, CPX_EDEVICE, "autotune: candidates disagree");
+    return CPX_OK;
+  };
+  double best_ms = 1e300, ms = 0;
+  SegShape best = c->shape;
+  for (const SegShape &s : cands) {
+    if (!seg_shape_supported(s) || seg_blocks_per_cu(s) < 1) continue;
+    c->shape = s;
+    int rc = trial(ms);
+    if (rc) return rc;
     if (ms < best_ms) { best_ms = ms; best = s; }
   }
   c->shape = best;
+  if (c->opts.alpha <= 0) {
+    double best_a = c->alpha_scale;
+    best_ms = 1e300;
+    for (double a : scales) {
+      c->alpha_scale = a;
+      int rc = trial(ms);
+      if (rc) return rc;
+      if (ms < best_ms) { best_ms = ms; best_a = a; }
+    }
+    c->alpha_scale = best_a;
+  }
+  {
+    uint32_t best_c = c->opts.c;
+    best_ms = 1e300;
+    for (uint32_t w : wheels) {
+      c->opts.c = w;
+      int rc = trial(ms);
+      if (rc) return rc;
+      if (ms < best_ms) { best_ms = ms; best_c = w; }
+    }
+    c->opts.c = best_c;
+  }
   c->shape_tuned = true;
   c->generation++;
   if (threads) *threads = (uint32_t)best.threads;
   if (kwords) *kwords = (uint32_t)best.kwords;
+  return CPX_OK;
+}
+
+int cpx_get_tuning(cpx_ctx *c, uint32_t *threads, uint32_t *kwords, double *alpha_scale, uint32_t *wheel_c)
+{
+  CTX_GUARD(c);
+  if (threads) *threads = (uint32_t)c->shape.threads;
+  if (kwords) *kwords = (uint32_t)c->shape.kwords;
+  if (alpha_scale) *alpha_scale = c->alpha_scale;
+  if (wheel_c) *wheel_c = c->opts.c;
   return CPX_OK;
 }
 

@@ -119,7 +119,7 @@ static double alpha_model(u128 x)
 uint64_t choose_y_host(const cpx_ctx *c, u128 x)
 {
   uint64_t cb = cpx_icbrt128(x), sq = cpx_isqrt128(x);
-  double alpha = c->opts.alpha > 0 ? c->opts.alpha : alpha_model(x);
+  double alpha = c->opts.alpha > 0 ? c->opts.alpha : alpha_model(x) * c->alpha_scale;
   long double yy = (long double)alpha * (long double)cb;
   uint64_t y = (uint64_t)yy;
   if (y < cb) y = cb;

@@ -82,6 +82,7 @@ struct cpx_ctx {
   // cpx_autotune may replace it for a batch
   SegShape shape{512, 32};
   bool shape_tuned = false;
+  double alpha_scale = 1.0;                     // factor on the alpha model (autotuned)
 
   // captured graph of the batch launch sequence
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};   // pre-terms, segments, post

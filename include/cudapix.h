@@ -173,9 +173,15 @@ int cpx_pi_batch_terms(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out
                        cpx_terms *terms, const cpx_batch_opts *opts, cpx_batch_stats *stats);
 
 /* ---- autotuner ----------------------------------------------------------- */
-/* Time the segment-kernel shapes on a calibration batch and keep the fastest
- * for this context (also run by cpx_pi_batch when opts->tune is set). */
+/* Time a calibration batch under each candidate and keep the fastest for this
+ * context, in three passes: the segment-kernel shape (threads x words), a
+ * factor on the built-in alpha(x) model (unless opts.alpha fixes alpha), and
+ * the number of wheel primes c (6..8); every candidate must return the same
+ * pi(x) for every query.  Also run by cpx_pi_batch when opts->tune is set. */
 int cpx_autotune(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, uint32_t *threads, uint32_t *kwords);
+/* The context's current tuning (any pointer may be NULL). */
+int cpx_get_tuning(cpx_ctx *ctx, uint32_t *threads, uint32_t *kwords, double *alpha_scale,
+                   uint32_t *wheel_c);
 
 /* ---- serving (continuous batching) --------------------------------------- */
 typedef struct {
