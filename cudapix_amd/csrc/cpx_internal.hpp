@@ -91,13 +91,24 @@ __host__ __device__ __forceinline__ uint32_t cpx_fastmod32(uint32_t a, uint32_t 
   return (uint32_t)(((u128)low * p) >> 64);
 }
 
-// Work tile of the persistent segment kernel.  A leaf-heavy hard segment is
-// split into nrep replica tiles: each sieves the whole segment but evaluates
-// only the leaves with index = rep (mod nrep), writing its own sum(mu) row.
-enum { TILE_HARD = 0 };
+// Queries of a batch that share one sieve of each segment: a tile sieves its
+// segment once for up to CPX_GROUP queries of similar x (same wheel), and
+// evaluates every member's leaves, counters and P2 counts against it.
+#ifndef CPX_GROUP
+#define CPX_GROUP 4
+#endif
+struct GroupDev {
+  uint32_t n;                   // members, sorted by x
+  uint32_t q[CPX_GROUP];        // query indices
+};
+
+// Work tile of the persistent segment kernel: one segment of one group.  A
+// leaf-heavy segment is split into nrep replica tiles: each sieves the whole
+// segment but evaluates only the leaves with index = rep (mod nrep), writing
+// its own sum(mu) rows.
 struct TileDev {
-  uint32_t query;
-  uint16_t kind;
+  uint32_t group;
+  uint16_t nrep;
   uint16_t rep;
   uint64_t seg;
 };
@@ -142,6 +153,7 @@ struct BatchDev {
   QueryDev *queries;     uint32_t *nqueries;
   StageDev *stages;
   TileDev *tiles;        uint32_t *ntiles;    uint32_t *tile_head;
+  GroupDev *groups;
   BlockDesc *ord_desc;   uint32_t *n_ord;
   BlockDesc *triv_desc;  uint32_t *n_triv;
   BlockDesc *easy_desc;  uint32_t *n_easy;

@@ -304,10 +304,16 @@ static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint
 struct QPlan {
   QueryDev d{};
   std::vector<BlockDesc> ord, triv, easy, fin;
-  std::vector<HardRowInfo> rowinfo;
+  std::vector<HardRowInfo> rowinfo;  // replicas and M rows are set per group
+  std::vector<float> rleaves;        // estimated leaves of each hard row
+  std::vector<float> rleafy;         // estimated leafy stages of each hard row
+  uint64_t parts = 0, tm = 0, mm = 0;
+};
+
+// Tiles of one group of queries (see GroupDev).
+struct GPlan {
   std::vector<TileDev> tiles;
   std::vector<uint16_t> tbucket;     // cost bucket of each tile (longest first)
-  uint64_t parts = 0, tm = 0, mm = 0;
 };
 
 // Cost buckets: 1/8 octave of the tile's cost estimate.
@@ -388,22 +394,21 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   for (uint64_t b = 0; b < d.nstage; b++) qp.fin.push_back({i, part++, b, b + 1});
   d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
   qp.parts = part;
-  // hard tiles: active stages per segment and, for the low (leaf-dense)
-  // segments, an estimate of the leaves to evaluate; a segment with many
-  // leaves becomes several replica tiles that split its leaves.
+  // hard rows: active stages per segment and, for the low (leaf-dense)
+  // segments, an estimate of the leaves to evaluate (replicas are chosen per
+  // group, from the members' leaves of the segment)
   std::vector<uint64_t> xps;
   if (d.hard_nseg) {
     xps.resize(d.nstage);
     for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (uint64_t)(h.x / c->h_primes[h.c + k]);
   }
-  const double leaf_target = 256.0 * T;                // leaf evaluations per replica
+  const double leaf_target = 256.0 * T;
   uint32_t sparse_run = 0;
-  uint64_t mrows = 0, toff = 0, moff = 0;
-  uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
+  uint64_t toff = 0;
   uint64_t act = d.nstage;                             // non-increasing in the segment
   qp.rowinfo.reserve(d.hard_nseg);
-  qp.tiles.reserve(d.hard_nseg + d.p2_nseg);
-  qp.tbucket.reserve(d.hard_nseg + d.p2_nseg);
+  qp.rleaves.reserve(d.hard_nseg);
+  qp.rleafy.reserve(d.hard_nseg);
   for (uint64_t s = 0; s < d.hard_nseg; s++) {
     const uint64_t lo = s * 2 * SB, hi = lo + 2 * SB;
     // active stages: p_b^2 <= x / lo, i.e. p_b^2 * lo <= x (the kernel's test)
@@ -431,30 +436,106 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
       }
     }
     if (est) sparse_run = leaves < leaf_target / 4 ? sparse_run + 1 : 0;
-    uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
-    qp.rowinfo.push_back({(uint32_t)mrows, nrep, (uint32_t)act, 0u, toff, moff});
-    mrows += nrep;
+    qp.rowinfo.push_back({0u, 0u, (uint32_t)act, 0u, toff, 0u});
+    qp.rleaves.push_back((float)leaves);
+    qp.rleafy.push_back((float)leafy);
     toff += act;
-    moff += (uint64_t)nrep * act;
-    const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + leafy * 1.5 +
-                        2.0 * leaves / ((double)nrep * T);
-    // replica 0 of a segment of the P2 range also crosses the primes from
+  }
+  qp.tm = toff;
+}
+
+// Groups of queries that share the segment sieve: queries with hard rows,
+// in increasing x, with the same wheel, at most CPX_GROUP, and every
+// member's P2 range (from p2_e) above every member's stage primes (segment 0
+// loses its stage primes to the sieve, so they must not be counted as P2
+// primes of another member).
+static void make_groups(const cpx_ctx *c, const std::vector<HostQuery> &hq,
+                        const std::vector<QPlan> &qps, std::vector<GroupDev> &groups)
+{
+  std::vector<uint32_t> hs;
+  for (uint32_t i = 0; i < (uint32_t)qps.size(); i++)
+    if (qps[i].d.hard_nseg) hs.push_back(i);
+  std::stable_sort(hs.begin(), hs.end(), [&](uint32_t a, uint32_t b) { return hq[a].x < hq[b].x; });
+  GroupDev cur{};
+  uint64_t smax = 0, p2min = ~0ull;
+  for (uint32_t i : hs) {
+    const QueryDev &d = qps[i].d;
+    const uint64_t si = d.nstage ? c->h_primes[d.c + d.nstage - 1] : 0;
+    const uint64_t pi = d.p2_nseg ? d.p2_e : ~0ull;
+    if (cur.n && (cur.n == CPX_GROUP || qps[cur.q[0]].d.c != d.c ||
+                  std::max(smax, si) > std::min(p2min, pi))) {
+      groups.push_back(cur);
+      cur = GroupDev{};
+      smax = 0;
+      p2min = ~0ull;
+    }
+    cur.q[cur.n++] = i;
+    smax = std::max(smax, si);
+    p2min = std::min(p2min, pi);
+  }
+  if (cur.n) groups.push_back(cur);
+}
+
+// Replicas, M rows and tiles of one group: a segment's replica count comes
+// from all members' leaves there; every member with the segment in its hard
+// range gets that many M rows.  Tile cost: the shared sieve once, each
+// member's leaves, and on replica 0 the P2 sieve and counts.
+static void plan_group(const cpx_ctx *c, const GroupDev &g, uint32_t gi, std::vector<QPlan> &qps,
+                       GPlan &gp)
+{
+  const uint64_t SB = c->shape.seg_bits();
+  const uint32_t T = (uint32_t)c->shape.threads;
+  const double leaf_target = 256.0 * T;
+  uint64_t nseg = 0;
+  for (uint32_t m = 0; m < g.n; m++) nseg = std::max(nseg, qps[g.q[m]].d.hard_nseg);
+  const uint32_t cw = qps[g.q[0]].d.c;
+  uint64_t mrows[CPX_GROUP] = {0}, moff[CPX_GROUP] = {0};
+  uint64_t nsp_i = 0;                                  // pi(sqrt(hi)), grows with s
+  gp.tiles.reserve(nseg);
+  gp.tbucket.reserve(nseg);
+  for (uint64_t s = 0; s < nseg; s++) {
+    const uint64_t hi = (s + 1) * 2 * SB;
+    double leaves = 0, lcost = 0;
+    uint32_t act = 0, np2 = 0;
+    for (uint32_t m = 0; m < g.n; m++) {
+      const QPlan &qp = qps[g.q[m]];
+      if (s >= qp.d.hard_nseg) continue;
+      leaves += qp.rleaves[s];
+      act = std::max(act, qp.rowinfo[s].act);
+      if (s >= qp.d.p2_seg0 && s < qp.d.p2_seg0 + qp.d.p2_nseg) np2++;
+    }
+    const uint32_t nrep = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(leaves / leaf_target)));
+    for (uint32_t m = 0; m < g.n; m++) {
+      QPlan &qp = qps[g.q[m]];
+      if (s >= qp.d.hard_nseg) continue;
+      HardRowInfo &ri = qp.rowinfo[s];
+      ri.nrep = nrep;
+      ri.mrow = (uint32_t)mrows[m];
+      ri.moff = moff[m];
+      mrows[m] += nrep;
+      moff[m] += (uint64_t)nrep * ri.act;
+      lcost += qp.rleafy[s] * 1.5 + 2.0 * qp.rleaves[s] / ((double)nrep * T);
+    }
+    const double cost = 1.0 + (double)act * (0.5 + 4.0 * SB / (64.0 * 1024.0 * T)) + lcost;
+    // replica 0 of a segment of a P2 range also crosses the primes from
     // p_(c+act+1) to sqrt(hi) and counts the P2 leaves there
     double p2cost = 0;
-    if (s >= d.p2_seg0 && s < d.p2_seg0 + d.p2_nseg) {
+    if (np2) {
       const double sq = std::min(std::sqrt((double)hi), (double)c->prime_limit);
       while (nsp_i < c->h_primes.size() && (double)c->h_primes[nsp_i] <= sq) nsp_i++;
-      const double nsp = (double)nsp_i - (double)(h.c + act);
-      p2cost = 2.0 + std::max(nsp, 0.0) * 8.0 / T;
+      const double nsp = (double)nsp_i - (double)(cw + act);
+      p2cost = 2.0 * np2 + std::max(nsp, 0.0) * 8.0 / T;
     }
     for (uint32_t r = 0; r < nrep; r++) {
-      qp.tiles.push_back({i, (uint16_t)TILE_HARD, (uint16_t)r, s});
-      qp.tbucket.push_back(cost_bucket(r == 0 ? cost + p2cost : cost));
+      gp.tiles.push_back({gi, (uint16_t)nrep, (uint16_t)r, s});
+      gp.tbucket.push_back(cost_bucket(r == 0 ? cost + p2cost : cost));
     }
   }
-  d.hard_nmrow = mrows;
-  qp.tm = toff;
-  qp.mm = moff;
+  for (uint32_t m = 0; m < g.n; m++) {
+    QPlan &qp = qps[g.q[m]];
+    qp.d.hard_nmrow = mrows[m];
+    qp.mm = moff[m];
+  }
 }
 
 // Run f(k) for k in [0, n) on up to `threads` host threads (inline for one).
@@ -490,6 +571,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   std::vector<QPlan> qps(nq);
   const unsigned nt = plan_threads(nq);
   parallel_for(nq, nt, [&](size_t i) { plan_query(c, hq[i], (uint32_t)i, mask, qps[i]); });
+  pl.groups.clear();
+  make_groups(c, hq, qps, pl.groups);
+  std::vector<GPlan> gps(pl.groups.size());
+  parallel_for(pl.groups.size(), nt,
+               [&](size_t g) { plan_group(c, pl.groups[g], (uint32_t)g, qps, gps[g]); });
   // place the shares: global offsets of every query's rows, parts and counters
   struct Off { size_t ord, triv, easy, fin, rows, tiles; };
   std::vector<Off> off(nq);
@@ -517,19 +603,20 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     pl.q[i] = d;
     off[i] = {nord, ntriv, neasy, nfin, nrows, ntiles};
     nord += qp.ord.size(); ntriv += qp.triv.size(); neasy += qp.easy.size();
-    nfin += qp.fin.size(); nrows += qp.rowinfo.size(); ntiles += qp.tiles.size();
+    nfin += qp.fin.size(); nrows += qp.rowinfo.size();
   }
+  for (const GPlan &gp : gps) ntiles += gp.tiles.size();
   pl.ord.resize(nord); pl.triv.resize(ntriv); pl.easy.resize(neasy);
   pl.fin.resize(nfin); pl.rowinfo.resize(nrows);
   // longest first: counting sort on the cost buckets (descending, stable in
-  // query order within a bucket), O(tiles)
+  // group order within a bucket), O(tiles)
   std::vector<uint32_t> start(PLAN_NB + 1, 0);
-  for (const QPlan &qp : qps)
-    for (uint16_t b : qp.tbucket) start[PLAN_NB - 1 - b + 1]++;
+  for (const GPlan &gp : gps)
+    for (uint16_t b : gp.tbucket) start[PLAN_NB - 1 - b + 1]++;
   for (int b = 0; b < PLAN_NB; b++) start[b + 1] += start[b];
   pl.tiles.resize(ntiles);
-  for (const QPlan &qp : qps)
-    for (size_t k = 0; k < qp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - qp.tbucket[k]]++] = qp.tiles[k];
+  for (const GPlan &gp : gps)
+    for (size_t k = 0; k < gp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - gp.tbucket[k]]++] = gp.tiles[k];
   parallel_for(nq, nt, [&](size_t i) {
     const QPlan &qp = qps[i];
     std::copy(qp.ord.begin(), qp.ord.end(), pl.ord.begin() + off[i].ord);
@@ -545,11 +632,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
 // Fixed-capacity layout: array offsets depend only on capacities, so the
 // pointers baked into a captured graph stay valid until a capacity grows.
 struct Caps {
-  uint64_t q, stages, tiles, ord, triv, easy, fin, tm, rows, p2rows, parts, mm, mrows;
+  uint64_t q, stages, tiles, groups, ord, triv, easy, fin, tm, rows, p2rows, parts, mm, mrows;
 };
 
 struct Layout {
-  size_t counters, queries, stages, tiles, ord, triv, easy, fin, rowinfo, plan_bytes;
+  size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, rowinfo, plan_bytes;
   size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
 };
 
@@ -563,6 +650,7 @@ static Layout make_layout(const Caps &k)
   L.queries = o;  o = al(o + k.q * sizeof(QueryDev));
   L.stages = o;   o = al(o + k.stages * sizeof(StageDev));
   L.tiles = o;    o = al(o + k.tiles * sizeof(TileDev));
+  L.groups = o;   o = al(o + k.groups * sizeof(GroupDev));
   L.ord = o;      o = al(o + k.ord * sizeof(BlockDesc));
   L.triv = o;     o = al(o + k.triv * sizeof(BlockDesc));
   L.easy = o;     o = al(o + k.easy * sizeof(BlockDesc));
@@ -587,7 +675,7 @@ static Layout make_layout(const Caps &k)
 static Caps needed_caps(const Plan &pl)
 {
   Caps k;
-  k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size();
+  k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size(); k.groups = pl.groups.size();
   k.ord = pl.ord.size(); k.triv = pl.triv.size(); k.easy = pl.easy.size(); k.fin = pl.fin.size();
   k.tm = pl.hard_tm; k.rows = pl.hard_rows; k.p2rows = pl.p2_rows; k.parts = pl.parts;
   k.mm = pl.hard_mm; k.mrows = pl.hard_mrows;
@@ -622,6 +710,7 @@ static int ensure_arena(cpx_ctx *c, const Plan &pl)
     if (n > cap) { cap = std::max<uint64_t>(n, cap + cap / 2); grow = true; }
   };
   fit(k.q, std::max<uint64_t>(need.q, 64)); fit(k.stages, need.stages); fit(k.tiles, need.tiles);
+  fit(k.groups, std::max<uint64_t>(need.groups, 16));
   fit(k.ord, need.ord); fit(k.triv, need.triv); fit(k.easy, need.easy); fit(k.fin, need.fin);
   fit(k.tm, need.tm); fit(k.rows, need.rows); fit(k.p2rows, need.p2rows); fit(k.parts, need.parts);
   fit(k.mm, need.mm); fit(k.mrows, need.mrows);
@@ -663,6 +752,7 @@ static BatchDev make_batchdev(cpx_ctx *c)
   b.queries = (QueryDev *)(P + L.queries);
   b.stages = (StageDev *)(P + L.stages);
   b.tiles = (TileDev *)(P + L.tiles);
+  b.groups = (GroupDev *)(P + L.groups);
   b.ord_desc = (BlockDesc *)(P + L.ord);
   b.triv_desc = (BlockDesc *)(P + L.triv);
   b.easy_desc = (BlockDesc *)(P + L.easy);
@@ -695,6 +785,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   };
   put(L.queries, pl.q.data(), pl.q.size() * sizeof(QueryDev));
   put(L.tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDev));
+  put(L.groups, pl.groups.data(), pl.groups.size() * sizeof(GroupDev));
   put(L.ord, pl.ord.data(), pl.ord.size() * sizeof(BlockDesc));
   put(L.triv, pl.triv.data(), pl.triv.size() * sizeof(BlockDesc));
   put(L.easy, pl.easy.data(), pl.easy.size() * sizeof(BlockDesc));
@@ -708,6 +799,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   HIPCHK(up(L.counters, 64));
   HIPCHK(up(L.queries, pl.q.size() * sizeof(QueryDev)));
   HIPCHK(up(L.tiles, pl.tiles.size() * sizeof(TileDev)));
+  HIPCHK(up(L.groups, pl.groups.size() * sizeof(GroupDev)));
   HIPCHK(up(L.ord, pl.ord.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.triv, pl.triv.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.easy, pl.easy.size() * sizeof(BlockDesc)));
@@ -800,9 +892,10 @@ uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &h)
     pairs += (r1 - r) * std::min(B, pv);
     r = r1;
   }
-  // T and M per pair (M rows of replicated segments add a little), rowinfo,
+  // T and M per pair (M rows of replicated segments add some: a group's
+  // leaf-dense segments give every member the group's replicas), rowinfo,
   // bact and the P2 rows, stage and carry descriptors, easy descriptors
-  double bytes = pairs * 8 * 1.1 + hseg * 48 + p2seg * 16 + 4096 + B * 80 + (h.y / 8192.0 + 16) * 48;
+  double bytes = pairs * 8 * 1.5 + hseg * 48 + p2seg * 16 + 4096 + B * 80 + (h.y / 8192.0 + 16) * 48;
   return (uint64_t)bytes;
 }
 

@@ -35,6 +35,7 @@ enum : uint32_t {
 struct Plan {
   std::vector<QueryDev> q;
   std::vector<TileDev> tiles;
+  std::vector<GroupDev> groups;
   std::vector<BlockDesc> ord, triv, easy, fin;
   std::vector<HardRowInfo> rowinfo;
   uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;

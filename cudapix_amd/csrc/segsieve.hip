@@ -33,7 +33,28 @@
 // stage up to sqrt(hi)) and the P2 counts; see p2_epilogue.
 #include "cpx_device.hpp"
 
-#define HARD_SCH 128       // stages per chunk of the hard stage table
+#define HARD_SCH 128                       // items (stage x member) per chunk
+#define HARD_SST (HARD_SCH / CPX_GROUP)    // stages per chunk
+#ifndef P2_WAVE_P
+#define P2_WAVE_P 1024     // P2 sieve: primes below this are crossed by a whole wave
+#endif
+static_assert(CPX_GROUP == 1 || CPX_GROUP == 2 || CPX_GROUP == 4 || CPX_GROUP == 8,
+              "a chunk's leafy mask word holds whole stages");
+
+// A group member's share of the current segment (zero when its hard range
+// ends before the segment).
+struct GroupMember {
+  uint64_t y;
+  uint64_t stage_off;   // its stage table
+  uint64_t toff;        // hard_T index of its stage 0 in this segment's row
+  uint64_t moff;        // hard_M index of its stage 0 for this replica
+  uint64_t acc_slot;    // hard_acc index for this replica
+  uint64_t row;         // its hard row (hard_bact)
+  uint32_t nst;         // its active stages in the segment
+  uint32_t on;          // the segment is in its hard range
+  uint32_t p2;          // ... and in its P2 range
+  uint32_t pad;
+};
 
 template <int T, int K>
 struct SegShared {
@@ -41,21 +62,23 @@ struct SegShared {
   uint32_t wtot[T / CPX_WAVE];   // wave totals
   int32_t wsum[T / CPX_WAVE];    // per-wave sums (P2)
   unsigned long long red_u[T / CPX_WAVE];
-  long long red_s[T / CPX_WAVE];
+  long long red_s[CPX_GROUP][T / CPX_WAVE];
+  GroupMember gm[CPX_GROUP];
   union {
-    struct {                     // hard: stage table and per-wave counters of the chunk
-      uint64_t xp[HARD_SCH];     // floor(x / p)
-      uint32_t p[HARD_SCH];
-      float rinv[HARD_SCH];      // 1 / p
-      uint32_t oseg[HARD_SCH];   // first odd multiple of p in the segment (bit offset)
+    struct {                     // hard: chunk table and per-wave counters
+      // per item i = k * CPX_GROUP + g (stage k of the chunk, member g)
+      uint64_t xp[HARD_SCH];     // floor(x_g / p_k)
       uint32_t cbase[HARD_SCH];  // leaf candidates: mtop/2 (dense) or first prime index
       uint32_t ccnt[HARD_SCH];   // candidates | HARD_WIDE | HARD_SPARSE
-      int32_t wm[HARD_SCH][T / CPX_WAVE];    // sum of mu of each wave's leaves per stage
-      uint16_t dw[HARD_SCH][T / CPX_WAVE];   // survivors each wave removes per stage
-      uint16_t wb[HARD_SCH][T / CPX_WAVE];   // first candidate of each wave (non-wide stages)
+      int32_t wm[HARD_SCH][T / CPX_WAVE];    // sum of mu of each wave's leaves
+      uint16_t wb[HARD_SCH][T / CPX_WAVE];   // first candidate of each wave (non-wide items)
+      uint32_t lmask[HARD_SCH / 32];         // items with candidates
+      // per stage k
+      uint32_t p[HARD_SST];
+      float rinv[HARD_SST];      // 1 / p
+      uint32_t oseg[HARD_SST];   // first odd multiple of p in the segment (bit offset)
+      uint16_t dw[HARD_SST][T / CPX_WAVE];   // survivors each wave removes
       uint32_t c0[T / CPX_WAVE]; // each wave's survivors at the chunk start
-      uint32_t lmask[HARD_SCH / 32];   // stages with candidates
-      uint32_t cend;             // first inactive stage of the chunk (or ~0)
     } st;
   } u;
   uint32_t tile;
@@ -148,81 +171,87 @@ __device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase
 #define HARD_WIDE 0x40000000u
 #define HARD_SPARSE 0x80000000u
 
-// Stage table of one chunk: for its stages k < clen (stage s0 + k) the
-// threads write p, 1/p, x/p, the first odd multiple of p in the segment (bit
-// offset), the leaf candidate range, a leafy bit, and for every wave w > 0
-// the boundary b_w of its candidates (wave w owns the leaves with n = x/(p m)
-// in [lo_w, lo_w + 4*32K*64)): for dense stages (m = mtop - 2i, n grows with
-// i) b_w counts the candidates with n < lo_w and is the wave's first index;
-// for sparse stages (m = the i-th prime, n falls with i) it counts those with
-// n >= lo_w and is the wave's end.  Clears the per-wave sums of mu; the first
-// inactive stage (p^2 > x/lo) lowers st.cend.  Caller syncs before and after.
+// Table of one chunk of stages [s0, s0 + clen) (all members share the stage
+// primes p_k = primes[c + s0 + k]).  Per stage: p, 1/p and the first odd
+// multiple of p in the segment (bit offset).  Per item (stage k, member g)
+// active for g (s0 + k below its active stages): x_g/p, the leaf candidate
+// range, a leafy bit, and for every wave w > 0 the boundary b_w of its
+// candidates (wave w owns the leaves with n = x/(p m) in
+// [lo_w, lo_w + 4*32K*64)): for dense items (m = mtop - 2i, n grows with i)
+// b_w counts the candidates with n < lo_w and is the wave's first index; for
+// sparse items (m = the i-th prime, n falls with i) it counts those with
+// n >= lo_w and is the wave's end.  Clears the per-wave sums of mu.  Caller
+// syncs before and after.
 template <int T, int K>
-__device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const QueryDev &q,
-                                                 const StageDev *st, uint32_t s0, uint32_t clen,
-                                                 uint64_t lo, uint64_t hi, uint64_t xlo,
-                                                 double inv_lo, double inv_hi,
-                                                 SegShared<T, K> &sh)
+__device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c, uint32_t s0,
+                                                 uint32_t clen, uint64_t lo, uint64_t hi,
+                                                 double inv_lo, double inv_hi, SegShared<T, K> &sh)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
+  constexpr uint32_t G = CPX_GROUP;
   constexpr uint64_t WB = (uint64_t)CPX_WAVE * 32 * K;
-  const uint64_t y = q.y;
-  for (uint32_t k = threadIdx.x; k < clen * NW; k += T) (&sh.u.st.wm[0][0])[k] = 0;
+  const uint32_t nit = clen * G;
+  for (uint32_t k = threadIdx.x; k < nit * NW; k += T) (&sh.u.st.wm[0][0])[k] = 0;
   for (uint32_t k = threadIdx.x; k < clen; k += T) {
-    const StageDev sd = st[s0 + k];
-    const uint64_t p = sd.p, xp = sd.xp;
-    if (p * p > xlo) {                       // inactive: so are all later stages
-      atomicMin(&sh.u.st.cend, k);
-      continue;
-    }
+    const uint64_t p = bd.primes[c + s0 + k];
     sh.u.st.p[k] = (uint32_t)p;
     sh.u.st.rinv[k] = 1.0f / (float)p;
-    sh.u.st.xp[k] = xp;
     // first odd multiple of p at or above lo + 1, as a bit offset (< p)
     const uint64_t start = lo + 1;
-    const uint64_t rr = start - cpx_div_recip(start, p, sd.inv_p) * p;
+    const uint64_t rr = start - cpx_div_recip(start, p, 1.0 / (double)p) * p;
     uint64_t mm = start + (rr ? p - rr : 0);
     if (!(mm & 1)) mm += p;
     sh.u.st.oseg[k] = (uint32_t)((mm - start) >> 1);
-    // m range of this segment's leaves of p: x/(p hi) < m <= x/(p lo), m > y/p
-    const bool sparse = p * p > y;
-    uint64_t mhi = y;
-    if (lo != 0 && (xp >> 50) < lo) {
-      const uint64_t v = cpx_div_recip(xp, lo, inv_lo);
-      if (v < mhi) mhi = v;
-    }
-    uint64_t mlo = ~0ull;
-    if ((xp >> 50) < hi) mlo = cpx_div_recip(xp, hi, inv_hi);
+  }
+  for (uint32_t it = threadIdx.x; it < nit; it += T) {
+    const uint32_t k = it / G, g = it % G;
+    const GroupMember &m = sh.gm[g];
     uint32_t ncand = 0, cbase = 0;
-    if (mlo < mhi) {
-      if (mlo < sd.ylo) mlo = sd.ylo;
-      if (sparse && mlo < p) mlo = p;
-      if (mhi > mlo) {
-        if (!sparse) {
-          const uint64_t mtop = mhi - ((mhi & 1) ^ 1);            // largest odd <= mhi
-          ncand = mtop > mlo ? (uint32_t)((mtop - mlo + 1) / 2) : 0;
-          cbase = (uint32_t)(mtop >> 1);
-        } else {
-          const uint64_t il = pi_lookup(bd.pitab, mlo);
-          ncand = (uint32_t)(pi_lookup(bd.pitab, mhi) - il);
-          cbase = (uint32_t)il;
+    bool sparse = false;
+    uint64_t xp = 0;
+    if (s0 + k < m.nst) {
+      const StageDev sd = bd.stages[m.stage_off + s0 + k];
+      const uint64_t p = sd.p, y = m.y;
+      xp = sd.xp;
+      // m range of this segment's leaves of p: x/(p hi) < m <= x/(p lo), m > y/p
+      sparse = p * p > y;
+      uint64_t mhi = y;
+      if (lo != 0 && (xp >> 50) < lo) {
+        const uint64_t v = cpx_div_recip(xp, lo, inv_lo);
+        if (v < mhi) mhi = v;
+      }
+      uint64_t mlo = ~0ull;
+      if ((xp >> 50) < hi) mlo = cpx_div_recip(xp, hi, inv_hi);
+      if (mlo < mhi) {
+        if (mlo < sd.ylo) mlo = sd.ylo;
+        if (sparse && mlo < p) mlo = p;
+        if (mhi > mlo) {
+          if (!sparse) {
+            const uint64_t mtop = mhi - ((mhi & 1) ^ 1);            // largest odd <= mhi
+            ncand = mtop > mlo ? (uint32_t)((mtop - mlo + 1) / 2) : 0;
+            cbase = (uint32_t)(mtop >> 1);
+          } else {
+            const uint64_t il = pi_lookup(bd.pitab, mlo);
+            ncand = (uint32_t)(pi_lookup(bd.pitab, mhi) - il);
+            cbase = (uint32_t)il;
+          }
         }
       }
     }
-    sh.u.st.cbase[k] = cbase;
-    sh.u.st.ccnt[k] = ncand | (sparse ? HARD_SPARSE : 0u) | (ncand > 0xffffu ? HARD_WIDE : 0u);
-    if (ncand) atomicOr(&sh.u.st.lmask[k >> 5], 1u << (k & 31));
+    sh.u.st.xp[it] = xp;
+    sh.u.st.cbase[it] = cbase;
+    sh.u.st.ccnt[it] = ncand | (sparse ? HARD_SPARSE : 0u) | (ncand > 0xffffu ? HARD_WIDE : 0u);
+    if (ncand) atomicOr(&sh.u.st.lmask[it >> 5], 1u << (it & 31));
   }
   if (NW == 1) return;
   __syncthreads();
   // n >= lo_w <=> m <= M = floor(x/p / lo_w)
-  const uint32_t cend = min(clen, sh.u.st.cend);
-  for (uint32_t idx = threadIdx.x; idx < cend * NW; idx += T) {
-    const uint32_t k = idx / NW, w = idx % NW;
-    const uint32_t cc = sh.u.st.ccnt[k];
+  for (uint32_t idx = threadIdx.x; idx < nit * NW; idx += T) {
+    const uint32_t it = idx / NW, w = idx % NW;
+    const uint32_t cc = sh.u.st.ccnt[it];
     if (w == 0 || !(cc & HARD_NC_MASK) || (cc & HARD_WIDE)) continue;
-    const uint32_t ncand = cc & HARD_NC_MASK, cb = sh.u.st.cbase[k];
-    const uint64_t xp = sh.u.st.xp[k];
+    const uint32_t ncand = cc & HARD_NC_MASK, cb = sh.u.st.cbase[it];
+    const uint64_t xp = sh.u.st.xp[it];
     const uint64_t lo_w = lo + 2 * (uint64_t)w * WB;          // > 0
     const uint64_t M = cpx_div_recip(xp, lo_w, 1.0 / (double)lo_w);
     uint64_t b;
@@ -230,14 +259,15 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, const Query
       const uint64_t mtop = 2 * (uint64_t)cb + 1;            // m = mtop - 2i
       b = mtop > M ? (mtop - M + 1) / 2 : 0;
     } else {
+      const uint64_t y = sh.gm[it % G].y;
       const uint64_t pm = pi_lookup(bd.pitab, M < y ? M : y);
       b = pm > cb ? pm - cb : 0;
     }
-    sh.u.st.wb[k][w] = (uint16_t)(b < ncand ? b : ncand);
+    sh.u.st.wb[it][w] = (uint16_t)(b < ncand ? b : ncand);
   }
 }
 
-// First leafy stage of the chunk at or after k (HARD_SCH if none).
+// First leafy item of the chunk at or after item k (HARD_SCH if none).
 template <int T, int K>
 __device__ __forceinline__ uint32_t hard_next_leafy(const SegShared<T, K> &sh, uint32_t k)
 {
@@ -249,11 +279,11 @@ __device__ __forceinline__ uint32_t hard_next_leafy(const SegShared<T, K> &sh, u
   return HARD_SCH;
 }
 
-// This wave's candidate index range [ia, ib) of chunk stage k.  Stages with
-// 2^16 candidates or more derive it here: n = x/(p m) in [lo_w, hi_w) <=>
-// floor(xp / hi_w) < m <= floor(xp / lo_w).
+// This wave's candidate index range [ia, ib) of chunk item k (of a member
+// with this y).  Items with 2^16 candidates or more derive it here:
+// n = x/(p m) in [lo_w, hi_w) <=> floor(xp / hi_w) < m <= floor(xp / lo_w).
 template <int T, int K>
-__device__ __forceinline__ void hard_wave_range(const BatchDev &bd, const QueryDev &q,
+__device__ __forceinline__ void hard_wave_range(const BatchDev &bd, uint64_t y,
                                                 const SegShared<T, K> &sh, uint32_t k, uint32_t wv,
                                                 uint64_t lo_w, uint64_t hi_w, double inv_lo_w,
                                                 double inv_hi_w, uint32_t &ia, uint32_t &ib)
@@ -281,7 +311,6 @@ __device__ __forceinline__ void hard_wave_range(const BatchDev &bd, const QueryD
     ia = (uint32_t)min(a, (uint64_t)ncand);
     ib = (uint32_t)min(b, (uint64_t)ncand);
   } else {                                                   // primes[cb + i]
-    const uint64_t y = q.y;
     const uint64_t pl = pi_lookup(bd.pitab, Mhi < y ? Mhi : y);
     const uint64_t ph = pi_lookup(bd.pitab, Mlo < y ? Mlo : y);
     ia = (uint32_t)(pl > cb ? min(pl - cb, (uint64_t)ncand) : 0);
@@ -318,64 +347,71 @@ __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const Seg
   return c;
 }
 
-// End of a chunk of stages [c0, c1) (all threads call; `running` is kept by
-// wave 0): writes T_b = survivors of the segment before stage b (replica 0)
-// and this replica's sum of mu per stage, and adds to acc the part of every
-// leaf's phi that the in-wave count left out -- the survivors of the earlier
-// waves at that stage:
-//   sum_k sum_w wm[k][w] * (A_w - sum_{j<k} sum_{w'<w} dw[j][w'])
+// End of a chunk of stages [s0, s0 + n) (all threads call; `running` is kept
+// by wave 0): writes, for every member g and its active stages b, T_b =
+// survivors of the segment before stage b (replica 0) and this replica's sum
+// of mu, and adds to acc[g] the part of every leaf's phi that the in-wave
+// count left out -- the survivors of the earlier waves at that stage:
+//   sum_k sum_w wm[k,g][w] * (A_w - sum_{j<k} sum_{w'<w} dw[j][w'])
 // with A_w the earlier waves' survivors at the chunk start.  Then resets the
-// chunk's leafy mask and end marker.
+// chunk's leafy mask.
 template <int T, int K>
-__device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uint32_t c1,
-                                           uint32_t &running, uint32_t *outT, int32_t *outM,
-                                           bool write_T, int64_t &acc)
+__device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &sh, uint32_t s0,
+                                           uint32_t n, uint32_t &running, bool write_T,
+                                           int64_t (&acc)[CPX_GROUP])
 {
   constexpr uint32_t NW = T / CPX_WAVE;
+  constexpr uint32_t G = CPX_GROUP;
   const uint32_t lane = lane_id(), wv = wave_id();
-  const uint32_t n = c1 - c0;
   __syncthreads();
   if (NW > 1) {
     uint32_t A = 0;
 #pragma unroll
     for (uint32_t w = 0; w < NW; w++) A += w < wv ? sh.u.st.c0[w] : 0u;
     uint32_t carry = 0;
-    int64_t corr = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
       const uint32_t k = k0 + lane;
-      uint32_t G = 0;
-      int32_t m = 0;
+      uint32_t Gk = 0;
       if (k < n) {
 #pragma unroll
-        for (uint32_t w = 0; w < NW; w++) G += w < wv ? (uint32_t)sh.u.st.dw[k][w] : 0u;
-        m = sh.u.st.wm[k][wv];
+        for (uint32_t w = 0; w < NW; w++) Gk += w < wv ? (uint32_t)sh.u.st.dw[k][w] : 0u;
       }
-      const uint32_t incl = wave_incl_scan(G);
-      corr += (int64_t)m * ((int64_t)A - (int64_t)(carry + incl - G));
+      const uint32_t incl = wave_incl_scan(Gk);
+      const int64_t D = (int64_t)A - (int64_t)(carry + incl - Gk);
+      if (k < n) {
+#pragma unroll
+        for (uint32_t g = 0; g < G; g++) acc[g] += (int64_t)sh.u.st.wm[k * G + g][wv] * D;
+      }
       carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
     }
-    acc += corr;
   }
-  if (wv == 0) {                       // T_b and the sum of mu per stage
+  if (wv == 0) {                       // T_b and the sums of mu per stage
     for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
       const uint32_t k = k0 + lane;
       uint32_t d = 0;
-      int32_t ms = 0;
       if (k < n) {
 #pragma unroll
-        for (uint32_t w = 0; w < NW; w++) { d += sh.u.st.dw[k][w]; ms += sh.u.st.wm[k][w]; }
+        for (uint32_t w = 0; w < NW; w++) d += sh.u.st.dw[k][w];
       }
       const uint32_t incl = wave_incl_scan(d);
       if (k < n) {
-        if (write_T) outT[c0 + k] = running - (incl - d);
-        outM[c0 + k] = ms;
+        const uint32_t Tk = running - (incl - d);
+#pragma unroll
+        for (uint32_t g = 0; g < G; g++) {
+          const GroupMember &m = sh.gm[g];
+          if (s0 + k >= m.nst) continue;
+          int32_t ms = 0;
+#pragma unroll
+          for (uint32_t w = 0; w < NW; w++) ms += sh.u.st.wm[k * G + g][w];
+          if (write_T) bd.hard_T[m.toff + s0 + k] = Tk;
+          bd.hard_M[m.moff + s0 + k] = ms;
+        }
       }
       running -= (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
     }
   }
   __syncthreads();
   if (threadIdx.x < HARD_SCH / 32) sh.u.st.lmask[threadIdx.x] = 0;
-  if (threadIdx.x == 0) sh.u.st.cend = ~0u;
   __syncthreads();
 }
 
@@ -383,13 +419,14 @@ __device__ __forceinline__ void hard_flush(SegShared<T, K> &sh, uint32_t c0, uin
 // src/P2.cu:76-133), run by replica 0 of a hard tile after its stages: the
 // bitmap already holds [lo, hi) sieved by the wheel and the stage primes
 // p_(c+1) .. p_kfirst; the primes from index kfirst with p^2 < hi are
-// crossed here (one per thread, from p^2), which leaves exactly the primes
-// above p2_e.  Then for each P2 prime p in (max(y, x/hi), min(sqrt x, x/lo)]
-// (pi-table bits, block-strided) the count of primes in (p2_e, x/p] held in
-// this segment; per segment: primes in (p2_e, hi), number of p, sum of the
-// counts.  The carry is applied in the combine kernel.
+// crossed here (from p^2), which leaves exactly the primes above the largest
+// p2_e of the members.  Then, for each member whose P2 range holds the
+// segment and each of its P2 primes p in (max(y, x/hi), min(sqrt x, x/lo)]
+// (pi-table bits, block-strided), the count of primes in (p2_e, x/p] held in
+// this segment; per member and segment: primes in (p2_e, hi), number of p,
+// sum of the counts.  The carry is applied in the combine kernel.
 template <int T, int K>
-__device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const QueryDev &q, uint64_t seg,
+__device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const GroupDev &grp, uint64_t seg,
                                             uint64_t lo, uint32_t kfirst, uint32_t *lds,
                                             SegShared<T, K> &sh)
 {
@@ -398,14 +435,28 @@ __device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const QueryDev &
   const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
   const uint64_t SB = (uint64_t)BPT * T;
   const uint64_t hi = lo + 2 * SB;
-  const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
   const uint32_t *__restrict__ sp = bd.primes;
   const uint32_t np_all = (uint32_t)bd.nprimes;
 
   __syncthreads();
+  // primes below P2_WAVE_P hit the segment more than SB / P2_WAVE_P times:
+  // one prime per wave, its lanes taking every 64th multiple, so no lane
+  // walks a long chain alone
+  for (uint32_t k = kfirst + (uint32_t)__builtin_amdgcn_readfirstlane((int)wv); k < np_all; k += NW) {
+    const uint32_t p = sp[k];
+    if (p >= P2_WAVE_P || (uint64_t)p * p >= hi) break;
+    const uint32_t o = (uint32_t)first_odd_multiple_bit_r(lo, p, (uint64_t)p * p);
+    // o < SB + p and 64 p < 2^16: no 32-bit overflow
+    for (uint32_t j = o + lane * p; j < (uint32_t)SB; j += CPX_WAVE * p) {
+      const uint32_t g = j >> 5;
+      atomicAnd(&lds[g + g / K], ~(1u << (j & 31)));
+    }
+  }
+  // the rest: one prime per lane
   for (uint32_t k = kfirst + t; k < np_all; k += T) {
     const uint32_t p = sp[k];
     if ((uint64_t)p * p >= hi) break;
+    if (p < P2_WAVE_P) continue;
     const uint64_t o = first_odd_multiple_bit_r(lo, p, (uint64_t)p * p);
     if (o >= SB) continue;
     for (uint32_t j = (uint32_t)o; j < (uint32_t)SB; j += p) {   // SB + p < 2^32
@@ -422,99 +473,118 @@ __device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const QueryDev &
   __syncthreads();
   uint32_t wp[NW];
   const uint32_t tot = read_wave_prefix<T, K>(sh, wp);
-  // survivors at or below p2_e (1, the stage primes and what is below
-  // sqrt x are not counted): only the first segment of the range has any
-  const uint32_t ce = q.p2_e >= lo ? count_below<T, K>(lds, sh, wp, (uint32_t)min((q.p2_e - lo + 1) >> 1, SB)) : 0u;
+  for (uint32_t g = 0; g < grp.n; g++) {
+    if (!sh.gm[g].p2) continue;
+    const QueryDev &q = bd.queries[grp.q[g]];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    // survivors at or below p2_e (1, the stage primes and what is below
+    // sqrt x are not counted): only the first segment of the range has any
+    const uint32_t ce = q.p2_e >= lo ? count_below<T, K>(lds, sh, wp, (uint32_t)min((q.p2_e - lo + 1) >> 1, SB)) : 0u;
 
-  unsigned long long loc = 0;
-  uint32_t np = 0;
-  uint64_t pmax = q.sqrtx;
-  if (lo) {
-    const uint64_t v = cpx_div128_to64(x, lo);
-    if (v < pmax) pmax = v;
-  }
-  uint64_t pmin = cpx_div128_to64(x, hi);
-  if (pmin < q.y) pmin = q.y;
-  if (pmax > pmin) {
-    const uint64_t e_lo = (pmin + 1) >> 6, e_hi = pmax >> 6;
-    for (uint64_t e = e_lo + t; e <= e_hi; e += T) {
-      const PiEntry pe = bd.pitab[e];
-      uint32_t b = pe.bits;
-      const uint64_t base = 64 * e;
-      // keep odd integers base + 2i + 1 in (pmin, pmax]
-      if (pmax < base + 63) b &= cpx_lowmask32((uint32_t)((pmax - base + 1) >> 1));
-      if (pmin > base) {
-        const uint64_t kk = (pmin - base + 1) >> 1;     // bits for integers <= pmin
-        b = kk >= 32 ? 0u : (b & ~cpx_lowmask32((uint32_t)kk));
-      }
-      while (b) {
-        const uint32_t i = __builtin_ctz(b);
-        b &= b - 1;
-        const uint64_t p = base + 2 * i + 1;
-        const uint64_t v = cpx_div128_to64(x, p);       // lo <= v < hi
-        loc += (unsigned long long)(count_below<T, K>(lds, sh, wp, (uint32_t)((v - lo + 1) >> 1)) - ce);
-        np++;
+    unsigned long long loc = 0;
+    uint32_t np = 0;
+    uint64_t pmax = q.sqrtx;
+    if (lo) {
+      const uint64_t v = cpx_div128_to64(x, lo);
+      if (v < pmax) pmax = v;
+    }
+    uint64_t pmin = cpx_div128_to64(x, hi);
+    if (pmin < q.y) pmin = q.y;
+    if (pmax > pmin) {
+      const uint64_t e_lo = (pmin + 1) >> 6, e_hi = pmax >> 6;
+      for (uint64_t e = e_lo + t; e <= e_hi; e += T) {
+        const PiEntry pe = bd.pitab[e];
+        uint32_t b = pe.bits;
+        const uint64_t base = 64 * e;
+        // keep odd integers base + 2i + 1 in (pmin, pmax]
+        if (pmax < base + 63) b &= cpx_lowmask32((uint32_t)((pmax - base + 1) >> 1));
+        if (pmin > base) {
+          const uint64_t kk = (pmin - base + 1) >> 1;     // bits for integers <= pmin
+          b = kk >= 32 ? 0u : (b & ~cpx_lowmask32((uint32_t)kk));
+        }
+        while (b) {
+          const uint32_t i = __builtin_ctz(b);
+          b &= b - 1;
+          const uint64_t p = base + 2 * i + 1;
+          const uint64_t v = cpx_div128_to64(x, p);       // lo <= v < hi
+          loc += (unsigned long long)(count_below<T, K>(lds, sh, wp, (uint32_t)((v - lo + 1) >> 1)) - ce);
+          np++;
+        }
       }
     }
-  }
-  loc = wave_sum(loc);
-  const uint32_t npw = wave_sum(np);
-  if (lane == 0) { sh.red_u[wv] = loc; sh.wsum[wv] = (int32_t)npw; }
-  __syncthreads();
-  if (t == 0) {
-    unsigned long long L = 0;
-    uint32_t N = 0;
-    for (uint32_t w = 0; w < NW; w++) { L += sh.red_u[w]; N += (uint32_t)sh.wsum[w]; }
-    const uint64_t row = q.p2_row0 + (seg - q.p2_seg0);
-    bd.p2_cnt[row] = tot - ce;
-    bd.p2_np[row] = N;
-    bd.p2_loc[row] = L;
+    loc = wave_sum(loc);
+    const uint32_t npw = wave_sum(np);
+    if (lane == 0) { sh.red_u[wv] = loc; sh.wsum[wv] = (int32_t)npw; }
+    __syncthreads();
+    if (t == 0) {
+      unsigned long long L = 0;
+      uint32_t N = 0;
+      for (uint32_t w = 0; w < NW; w++) { L += sh.red_u[w]; N += (uint32_t)sh.wsum[w]; }
+      const uint64_t row = q.p2_row0 + (seg - q.p2_seg0);
+      bd.p2_cnt[row] = tot - ce;
+      bd.p2_np[row] = N;
+      bd.p2_loc[row] = L;
+    }
+    __syncthreads();
   }
 }
 
+// One tile: segment `seg` of a group, replica `rep` of `nrep`.  The wheel
+// and the stage primes sieve the segment once; every member's leaves are
+// evaluated against that one bitmap.
 template <int T, int K>
-__device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev &q,
-                                             const TileDev &tile, uint32_t *lds, SegShared<T, K> &sh)
+__device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &tile, uint32_t *lds,
+                                             SegShared<T, K> &sh)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
   constexpr uint32_t BPT = 32 * K;
   constexpr uint32_t WB = CPX_WAVE * BPT;                // bits per wave
+  constexpr uint32_t G = CPX_GROUP;
   const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
   const uint64_t SB = (uint64_t)BPT * T;
   const uint64_t seg = tile.seg;
   const uint64_t lo = seg * 2 * SB;
   const uint64_t hi = lo + 2 * SB;
-  const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
-  const WheelInfo wi = bd.winfo[q.c];
-  const uint64_t row = q.hard_row0 + seg;
-  const HardRowInfo ri = bd.rowinfo[row];
-  const uint32_t rep = tile.rep, nrep = ri.nrep;
-  const uint64_t mrow = (uint64_t)ri.mrow + rep;
-  // counters of the active stages only: the plan sized this row for ri.act
-  // stages, and the stage loop stops there at the latest
-  const uint32_t nst = min(q.nstage, ri.act);
-  uint32_t *outT = bd.hard_T + q.tm_off + ri.toff;
-  int32_t *outM = bd.hard_M + q.m_off + ri.moff + (uint64_t)rep * ri.act;
-  const StageDev *st = bd.stages + q.stage_off;
+  const GroupDev &grp = bd.groups[tile.group];
+  const uint32_t rep = tile.rep;
+  const uint32_t c = bd.queries[grp.q[0]].c;           // members share the wheel
+  const WheelInfo wi = bd.winfo[c];
+  // members' rows of this segment: counters of the active stages only (the
+  // plan sized each row for ri.act stages)
+  if (t < G) {
+    GroupMember m{};
+    if (t < grp.n) {
+      const QueryDev &q = bd.queries[grp.q[t]];
+      if (seg < q.hard_nseg) {
+        const uint64_t row = q.hard_row0 + seg;
+        const HardRowInfo ri = bd.rowinfo[row];
+        m.y = q.y;
+        m.stage_off = q.stage_off;
+        m.toff = q.tm_off + ri.toff;
+        m.moff = q.m_off + ri.moff + (uint64_t)rep * ri.act;
+        m.acc_slot = q.mrow_base + ri.mrow + rep;
+        m.row = row;
+        m.nst = min(q.nstage, ri.act);
+        m.on = 1;
+        m.p2 = seg >= q.p2_seg0 && seg < q.p2_seg0 + q.p2_nseg;
+      }
+    }
+    sh.gm[t] = m;
+  }
   // this wave's integers [lo_w, hi_w); this lane's first bit, for t*32K mod p
   const uint64_t lo_w = lo + 2 * (uint64_t)wv * WB, hi_w = lo_w + 2 * (uint64_t)WB;
   const double inv_lo_w = lo_w ? 1.0 / (double)lo_w : 0.0, inv_hi_w = 1.0 / (double)hi_w;
   const uint32_t b0 = t * BPT;                          // < 2^20
   const float fb0 = (float)b0;
-  const uint32_t cstep = CPX_WAVE * nrep;               // candidate stride of a lane
-
-  // stage b is active while p_b^2 <= x / lo (all leaves of p_b have n < x/p_b^2)
-  uint64_t xlo = ~0ull;
-  if (lo) {
-    u128 qq = x / lo;
-    xlo = (qq >> 64) ? ~0ull : (uint64_t)qq;
-  }
+  const uint32_t cstep = CPX_WAVE * (uint32_t)tile.nrep;   // candidate stride of a lane
   const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
 
   seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
   if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
-  if (t == 0) sh.u.st.cend = ~0u;
   __syncthreads();
+  uint32_t nst = 0, p2any = 0;                          // stages of the segment: the members' most
+#pragma unroll
+  for (uint32_t g = 0; g < G; g++) { nst = max(nst, sh.gm[g].nst); p2any |= sh.gm[g].p2; }
   uint32_t cnt_lo = own_count_lo<K>(lds, t), cnt = cnt_lo;   // survivors: first half, all
 #pragma unroll 8
   for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
@@ -522,82 +592,89 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
     uint32_t ws = wave_sum(cnt);
     if (lane == 0) sh.wsum[wv] = (int32_t)ws;
   }
-  uint32_t s0 = 0, clen = min((uint32_t)HARD_SCH, nst);
-  hard_chunk_table<T, K>(bd, q, st, s0, clen, lo, hi, xlo, inv_lo, inv_hi, sh);
+  uint32_t s0 = 0, clen = min((uint32_t)HARD_SST, nst);
+  if (clen) hard_chunk_table<T, K>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
   __syncthreads();
   uint32_t running = 0;                                 // survivors before the next stage
   for (uint32_t w = 0; w < NW; w++) running += (uint32_t)sh.wsum[w];
 
-  int64_t acc = 0;
-  uint32_t nact = 0;                                    // active stages of the segment
-  for (;;) {
-    const uint32_t cend = min(clen, sh.u.st.cend);
+  int64_t acc[G];
+#pragma unroll
+  for (uint32_t g = 0; g < G; g++) acc[g] = 0;
+  while (clen) {
     {
       const uint32_t ws = wave_sum(cnt);                // this wave's survivors at the chunk start
       if (lane == 0) sh.u.st.c0[wv] = ws;
     }
-    // one-ahead prefetch of this lane's first candidate of the next leafy stage
+    // one-ahead prefetch of this lane's first candidate of the next leafy item
     uint32_t pf_k = hard_next_leafy<T, K>(sh, 0);
     int32_t pf = 0;
-    if (pf_k < cend) {
+    if (pf_k < clen * G) {
       uint32_t ia, ib;
-      hard_wave_range<T, K>(bd, q, sh, pf_k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
+      hard_wave_range<T, K>(bd, sh.gm[pf_k % G].y, sh, pf_k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
       const uint32_t cc = sh.u.st.ccnt[pf_k];
       if (ia + rep * CPX_WAVE + lane < ib)
         pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc & HARD_SPARSE, ia + rep * CPX_WAVE + lane);
     }
     uint32_t lw = 0;
-    for (uint32_t k = 0; k < cend; k++) {
-      if ((k & 31) == 0) lw = sh.u.st.lmask[k >> 5];
+    for (uint32_t k = 0; k < clen; k++) {
+      const uint32_t it0 = k * G;
+      if ((it0 & 31) == 0) lw = sh.u.st.lmask[it0 >> 5];
       const uint32_t p = sh.u.st.p[k];
-      if ((lw >> (k & 31)) & 1u) {
+      const uint32_t lbits = (lw >> (it0 & 31)) & ((1u << G) - 1u);
+      if (lbits) {
         // ---- leaves of p whose n falls in this wave's bits (state: crossed
-        //      by p_1 .. p_{b-1}); the wave publishes its own prefix, no
-        //      other wave is involved
+        //      by p_1 .. p_{b-1}), member by member; the wave publishes its
+        //      own prefix, no other wave is involved
         {
           const uint32_t incl = wave_incl_scan(cnt);
           sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
-        const uint32_t cc = sh.u.st.ccnt[k], cb = sh.u.st.cbase[k];
-        const bool sparse = cc & HARD_SPARSE;
-        const uint64_t xp = sh.u.st.xp[k];
-        const double xpf = (double)xp;
-        uint32_t ia, ib;
-        hard_wave_range<T, K>(bd, q, sh, k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
-        int32_t smu = 0;
-        if (ib > ia) {
-          int32_t v = pf;
-          for (uint32_t i = ia + rep * CPX_WAVE + lane; i < ib; i += cstep) {
-            if (i != ia + rep * CPX_WAVE + lane) v = hard_fetch(bd, cb, sparse, i);
-            uint64_t m;
-            int32_t sg = -1;
-            if (sparse) {
-              m = (uint32_t)v;                              // prime, mu = -1
-            } else {
-              const uint32_t av = (uint32_t)(v < 0 ? -v : v);
-              if (av <= p) continue;                        // mu(m) = 0 or lpf(m) <= p
-              m = 2 * (uint64_t)(cb - i) + 1;
-              sg = v > 0 ? 1 : -1;
+#pragma unroll
+        for (uint32_t g = 0; g < G; g++) {
+          if (!((lbits >> g) & 1u)) continue;
+          const uint32_t it = it0 + g;
+          const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
+          const bool sparse = cc & HARD_SPARSE;
+          const uint64_t xp = sh.u.st.xp[it];
+          const double xpf = (double)xp;
+          uint32_t ia, ib;
+          hard_wave_range<T, K>(bd, sh.gm[g].y, sh, it, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
+          int32_t smu = 0;
+          if (ib > ia) {
+            int32_t v = pf;
+            for (uint32_t i = ia + rep * CPX_WAVE + lane; i < ib; i += cstep) {
+              if (i != ia + rep * CPX_WAVE + lane) v = hard_fetch(bd, cb, sparse, i);
+              uint64_t m;
+              int32_t sg = -1;
+              if (sparse) {
+                m = (uint32_t)v;                              // prime, mu = -1
+              } else {
+                const uint32_t av = (uint32_t)(v < 0 ? -v : v);
+                if (av <= p) continue;                        // mu(m) = 0 or lpf(m) <= p
+                m = 2 * (uint64_t)(cb - i) + 1;
+                sg = v > 0 ? 1 : -1;
+              }
+              const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo_w <= n < hi_w
+              const uint32_t cn = count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
+              if (sg > 0) { acc[g] += cn; smu++; } else { acc[g] -= cn; smu--; }
             }
-            const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo_w <= n < hi_w
-            const uint32_t c = count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
-            if (sg > 0) { acc += c; smu++; } else { acc -= c; smu--; }
+            // sum of mu over the wave (one candidate per lane at most: two ballots)
+            int32_t wm;
+            if (ib - ia <= cstep)
+              wm = (int32_t)__popcll(__ballot(smu > 0)) - (int32_t)__popcll(__ballot(smu < 0));
+            else
+              wm = (int32_t)wave_sum((uint32_t)smu);
+            if (lane == 0) sh.u.st.wm[it][wv] = wm;
           }
-          // sum of mu over the wave (one candidate per lane at most: two ballots)
-          int32_t wm;
-          if (ib - ia <= cstep)
-            wm = (int32_t)__popcll(__ballot(smu > 0)) - (int32_t)__popcll(__ballot(smu < 0));
-          else
-            wm = (int32_t)wave_sum((uint32_t)smu);
-          if (lane == 0) sh.u.st.wm[k][wv] = wm;
-        }
-        const uint32_t nk = hard_next_leafy<T, K>(sh, k + 1);
-        if (nk < cend) {
-          uint32_t na, nb;
-          hard_wave_range<T, K>(bd, q, sh, nk, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, na, nb);
-          if (na + rep * CPX_WAVE + lane < nb)
-            pf = hard_fetch(bd, sh.u.st.cbase[nk], sh.u.st.ccnt[nk] & HARD_SPARSE,
-                            na + rep * CPX_WAVE + lane);
+          const uint32_t nk = hard_next_leafy<T, K>(sh, it + 1);
+          if (nk < clen * G) {
+            uint32_t na, nb;
+            hard_wave_range<T, K>(bd, sh.gm[nk % G].y, sh, nk, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, na, nb);
+            if (na + rep * CPX_WAVE + lane < nb)
+              pf = hard_fetch(bd, sh.u.st.cbase[nk], sh.u.st.ccnt[nk] & HARD_SPARSE,
+                              na + rep * CPX_WAVE + lane);
+          }
         }
       }
       // ---- cross off the odd multiples of p in this thread's own bits,
@@ -642,25 +719,27 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const QueryDev 
         if (lane == 0) sh.u.st.dw[k][wv] = (uint16_t)wd;   // this wave's removals
       }
     }
-    nact = s0 + cend;
-    hard_flush<T, K>(sh, s0, nact, running, outT, outM, rep == 0, acc);
-    if (cend < clen || nact >= nst) break;
-    s0 = nact;
-    clen = min((uint32_t)HARD_SCH, nst - s0);
-    hard_chunk_table<T, K>(bd, q, st, s0, clen, lo, hi, xlo, inv_lo, inv_hi, sh);
-    __syncthreads();
+    hard_flush<T, K>(bd, sh, s0, clen, running, rep == 0, acc);
+    s0 += clen;
+    clen = min((uint32_t)HARD_SST, nst - s0);
+    if (clen) {
+      hard_chunk_table<T, K>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
+      __syncthreads();
+    }
   }
-  long long a = wave_sum((long long)acc);
-  if (lane == 0) sh.red_s[wv] = a;
+#pragma unroll
+  for (uint32_t g = 0; g < G; g++) {
+    const long long a = wave_sum((long long)acc[g]);
+    if (lane == 0) sh.red_s[g][wv] = a;
+  }
   __syncthreads();
-  if (t == 0) {
+  if (t < G && sh.gm[t].on) {
     long long tt = 0;
-    for (uint32_t w = 0; w < NW; w++) tt += sh.red_s[w];
-    bd.hard_acc[q.mrow_base + mrow] = tt;
-    if (rep == 0) bd.hard_bact[row] = nact;
+    for (uint32_t w = 0; w < NW; w++) tt += sh.red_s[t][w];
+    bd.hard_acc[sh.gm[t].acc_slot] = tt;
+    if (rep == 0) bd.hard_bact[sh.gm[t].row] = sh.gm[t].nst;
   }
-  if (rep == 0 && seg >= q.p2_seg0 && seg < q.p2_seg0 + q.p2_nseg)
-    p2_epilogue<T, K>(bd, q, seg, lo, q.c + nact, lds, sh);
+  if (rep == 0 && p2any) p2_epilogue<T, K>(bd, grp, seg, lo, c + nst, lds, sh);
 }
 
 template <int T, int K>
@@ -676,8 +755,7 @@ __global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
     const uint32_t ti = sh.tile;
     if (ti >= ntiles) break;
     const TileDev tile = bd.tiles[ti];
-    const QueryDev q = bd.queries[tile.query];
-    hard_segment<T, K>(bd, q, tile, lds, sh);
+    hard_segment<T, K>(bd, tile, lds, sh);
     __syncthreads();
   }
 }
