@@ -33,8 +33,8 @@
 // stage up to sqrt(hi)) and the P2 counts; see p2_epilogue.
 #include "cpx_device.hpp"
 
-#define HARD_SCH 128                       // items (stage x member) per chunk
-#define HARD_SST (HARD_SCH / CPX_GROUP)    // stages per chunk
+#define HARD_SCH 128       // items (stage x member slot) per chunk
+#define HARD_SST 64        // stages per chunk at most
 #ifndef P2_WAVE_P
 #define P2_WAVE_P 1024     // P2 sieve: primes below this are crossed by a whole wave
 #endif
@@ -66,7 +66,8 @@ struct SegShared {
   GroupMember gm[CPX_GROUP];
   union {
     struct {                     // hard: chunk table and per-wave counters
-      // per item i = k * CPX_GROUP + g (stage k of the chunk, member g)
+      // per item i = k * gs + g (stage k of the chunk, member g; gs member
+      // slots, a power of two)
       uint64_t xp[HARD_SCH];     // floor(x_g / p_k)
       uint32_t cbase[HARD_SCH];  // leaf candidates: mtop/2 (dense) or first prime index
       uint32_t ccnt[HARD_SCH];   // candidates | HARD_WIDE | HARD_SPARSE
@@ -184,13 +185,12 @@ __device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase
 // syncs before and after.
 template <int T, int K>
 __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c, uint32_t s0,
-                                                 uint32_t clen, uint64_t lo, uint64_t hi,
+                                                 uint32_t clen, uint32_t lgs, uint64_t lo, uint64_t hi,
                                                  double inv_lo, double inv_hi, SegShared<T, K> &sh)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
-  constexpr uint32_t G = CPX_GROUP;
   constexpr uint64_t WB = (uint64_t)CPX_WAVE * 32 * K;
-  const uint32_t nit = clen * G;
+  const uint32_t nit = clen << lgs;
   for (uint32_t k = threadIdx.x; k < nit * NW; k += T) (&sh.u.st.wm[0][0])[k] = 0;
   for (uint32_t k = threadIdx.x; k < clen; k += T) {
     const uint64_t p = bd.primes[c + s0 + k];
@@ -204,7 +204,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
     sh.u.st.oseg[k] = (uint32_t)((mm - start) >> 1);
   }
   for (uint32_t it = threadIdx.x; it < nit; it += T) {
-    const uint32_t k = it / G, g = it % G;
+    const uint32_t k = it >> lgs, g = it & ((1u << lgs) - 1u);
     const GroupMember &m = sh.gm[g];
     uint32_t ncand = 0, cbase = 0;
     bool sparse = false;
@@ -259,7 +259,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
       const uint64_t mtop = 2 * (uint64_t)cb + 1;            // m = mtop - 2i
       b = mtop > M ? (mtop - M + 1) / 2 : 0;
     } else {
-      const uint64_t y = sh.gm[it % G].y;
+      const uint64_t y = sh.gm[it & ((1u << lgs) - 1u)].y;
       const uint64_t pm = pi_lookup(bd.pitab, M < y ? M : y);
       b = pm > cb ? pm - cb : 0;
     }
@@ -347,6 +347,56 @@ __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const Seg
   return c;
 }
 
+// This wave's leaves of chunk stage k as one list over the stage's leafy
+// members (lbits): member g's candidates [ia[g], ia[g] + n_g) are list
+// entries [pre[g], pre[g] + n_g), pre[] non-decreasing.  Returns the list
+// length; everything here is wave-uniform.
+template <int T, int K>
+__device__ __forceinline__ uint32_t hard_stage_list(const BatchDev &bd, const SegShared<T, K> &sh,
+                                                    uint32_t k, uint32_t lgs, uint32_t lbits,
+                                                    uint32_t wv, uint64_t lo_w, uint64_t hi_w,
+                                                    double inv_lo_w, double inv_hi_w,
+                                                    uint32_t (&ia)[CPX_GROUP], uint32_t (&pre)[CPX_GROUP])
+{
+  uint32_t tot = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < CPX_GROUP; g++) {
+    pre[g] = tot;
+    ia[g] = 0;
+    if ((lbits >> g) & 1u) {
+      uint32_t a, b;
+      hard_wave_range<T, K>(bd, sh.gm[g].y, sh, (k << lgs) + g, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
+      a = (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
+      b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+      if (b > a) { ia[g] = a; tot += b - a; }
+    }
+  }
+  return tot;
+}
+
+// Member g and candidate index i of list entry f.
+__device__ __forceinline__ uint32_t hard_list_entry(uint32_t f, const uint32_t (&ia)[CPX_GROUP],
+                                                    const uint32_t (&pre)[CPX_GROUP], uint32_t &i)
+{
+  uint32_t g = 0;
+#pragma unroll
+  for (uint32_t h = 1; h < CPX_GROUP; h++) g += f >= pre[h] ? 1u : 0u;
+  uint32_t a = ia[0], b = 0;
+#pragma unroll
+  for (uint32_t h = 1; h < CPX_GROUP; h++)
+    if (g == h) { a = ia[h]; b = pre[h]; }
+  i = a + (f - b);
+  return g;
+}
+
+// Leafy-mask bits of chunk stage k (one per member slot).
+template <int T, int K>
+__device__ __forceinline__ uint32_t hard_stage_bits(const SegShared<T, K> &sh, uint32_t k, uint32_t lgs)
+{
+  const uint32_t it0 = k << lgs, gs = 1u << lgs;
+  return (sh.u.st.lmask[it0 >> 5] >> (it0 & 31)) & (gs == 32 ? ~0u : (1u << gs) - 1u);
+}
+
 // End of a chunk of stages [s0, s0 + n) (all threads call; `running` is kept
 // by wave 0): writes, for every member g and its active stages b, T_b =
 // survivors of the segment before stage b (replica 0) and this replica's sum
@@ -357,7 +407,7 @@ __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const Seg
 // chunk's leafy mask.
 template <int T, int K>
 __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &sh, uint32_t s0,
-                                           uint32_t n, uint32_t &running, bool write_T,
+                                           uint32_t n, uint32_t gs, uint32_t &running, bool write_T,
                                            int64_t (&acc)[CPX_GROUP])
 {
   constexpr uint32_t NW = T / CPX_WAVE;
@@ -380,7 +430,8 @@ __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &
       const int64_t D = (int64_t)A - (int64_t)(carry + incl - Gk);
       if (k < n) {
 #pragma unroll
-        for (uint32_t g = 0; g < G; g++) acc[g] += (int64_t)sh.u.st.wm[k * G + g][wv] * D;
+        for (uint32_t g = 0; g < G; g++)
+          if (g < gs) acc[g] += (int64_t)sh.u.st.wm[k * gs + g][wv] * D;
       }
       carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
     }
@@ -399,10 +450,10 @@ __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &
 #pragma unroll
         for (uint32_t g = 0; g < G; g++) {
           const GroupMember &m = sh.gm[g];
-          if (s0 + k >= m.nst) continue;
+          if (g >= gs || s0 + k >= m.nst) continue;
           int32_t ms = 0;
 #pragma unroll
-          for (uint32_t w = 0; w < NW; w++) ms += sh.u.st.wm[k * G + g][w];
+          for (uint32_t w = 0; w < NW; w++) ms += sh.u.st.wm[k * gs + g][w];
           if (write_T) bd.hard_T[m.toff + s0 + k] = Tk;
           bd.hard_M[m.moff + s0 + k] = ms;
         }
@@ -548,6 +599,10 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
   const GroupDev &grp = bd.groups[tile.group];
   const uint32_t rep = tile.rep;
   const uint32_t c = bd.queries[grp.q[0]].c;           // members share the wheel
+  // member slots gs = 2^lgs >= members; a chunk holds HARD_SCH / gs stages
+  const uint32_t lgs = grp.n <= 1 ? 0u : (uint32_t)(32 - __builtin_clz(grp.n - 1));
+  const uint32_t gs = 1u << lgs;
+  const uint32_t sst = min((uint32_t)HARD_SST, (uint32_t)HARD_SCH >> lgs);
   const WheelInfo wi = bd.winfo[c];
   // members' rows of this segment: counters of the active stages only (the
   // plan sized each row for ri.act stages)
@@ -592,8 +647,8 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
     uint32_t ws = wave_sum(cnt);
     if (lane == 0) sh.wsum[wv] = (int32_t)ws;
   }
-  uint32_t s0 = 0, clen = min((uint32_t)HARD_SST, nst);
-  if (clen) hard_chunk_table<T, K>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
+  uint32_t s0 = 0, clen = min(sst, nst);
+  if (clen) hard_chunk_table<T, K>(bd, c, s0, clen, lgs, lo, hi, inv_lo, inv_hi, sh);
   __syncthreads();
   uint32_t running = 0;                                 // survivors before the next stage
   for (uint32_t w = 0; w < NW; w++) running += (uint32_t)sh.wsum[w];
@@ -606,74 +661,80 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
       const uint32_t ws = wave_sum(cnt);                // this wave's survivors at the chunk start
       if (lane == 0) sh.u.st.c0[wv] = ws;
     }
-    // one-ahead prefetch of this lane's first candidate of the next leafy item
-    uint32_t pf_k = hard_next_leafy<T, K>(sh, 0);
+    // one-ahead prefetch: the next leafy stage's list and this lane's first
+    // candidate there
+    const uint32_t f0 = rep * CPX_WAVE + lane;          // this lane's first list entry
+    uint32_t nk = hard_next_leafy<T, K>(sh, 0) >> lgs;  // next leafy stage (>= clen: none)
+    uint32_t nia[G], npre[G], ntot = 0;
     int32_t pf = 0;
-    if (pf_k < clen * G) {
-      uint32_t ia, ib;
-      hard_wave_range<T, K>(bd, sh.gm[pf_k % G].y, sh, pf_k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
-      const uint32_t cc = sh.u.st.ccnt[pf_k];
-      if (ia + rep * CPX_WAVE + lane < ib)
-        pf = hard_fetch(bd, sh.u.st.cbase[pf_k], cc & HARD_SPARSE, ia + rep * CPX_WAVE + lane);
+    if (nk < clen) {
+      ntot = hard_stage_list<T, K>(bd, sh, nk, lgs, hard_stage_bits<T, K>(sh, nk, lgs), wv, lo_w, hi_w,
+                                   inv_lo_w, inv_hi_w, nia, npre);
+      if (f0 < ntot) {
+        uint32_t i;
+        const uint32_t it = (nk << lgs) + hard_list_entry(f0, nia, npre, i);
+        pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
+      }
     }
-    uint32_t lw = 0;
     for (uint32_t k = 0; k < clen; k++) {
-      const uint32_t it0 = k * G;
-      if ((it0 & 31) == 0) lw = sh.u.st.lmask[it0 >> 5];
+      const uint32_t it0 = k << lgs;
       const uint32_t p = sh.u.st.p[k];
-      const uint32_t lbits = (lw >> (it0 & 31)) & ((1u << G) - 1u);
-      if (lbits) {
+      if (k == nk) {
         // ---- leaves of p whose n falls in this wave's bits (state: crossed
-        //      by p_1 .. p_{b-1}), member by member; the wave publishes its
-        //      own prefix, no other wave is involved
+        //      by p_1 .. p_{b-1}), all members' in one list; the wave
+        //      publishes its own prefix, no other wave is involved
         {
           const uint32_t incl = wave_incl_scan(cnt);
           sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
+        int32_t wm[G];                                  // sum of mu of the wave's leaves per member
 #pragma unroll
-        for (uint32_t g = 0; g < G; g++) {
-          if (!((lbits >> g) & 1u)) continue;
+        for (uint32_t g = 0; g < G; g++) wm[g] = 0;
+        int32_t v = pf;
+        for (uint32_t f = f0; f < ntot; f += cstep) {
+          uint32_t i;
+          const uint32_t g = hard_list_entry(f, nia, npre, i);
           const uint32_t it = it0 + g;
           const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
           const bool sparse = cc & HARD_SPARSE;
-          const uint64_t xp = sh.u.st.xp[it];
-          const double xpf = (double)xp;
-          uint32_t ia, ib;
-          hard_wave_range<T, K>(bd, sh.gm[g].y, sh, it, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, ia, ib);
-          int32_t smu = 0;
-          if (ib > ia) {
-            int32_t v = pf;
-            for (uint32_t i = ia + rep * CPX_WAVE + lane; i < ib; i += cstep) {
-              if (i != ia + rep * CPX_WAVE + lane) v = hard_fetch(bd, cb, sparse, i);
-              uint64_t m;
-              int32_t sg = -1;
-              if (sparse) {
-                m = (uint32_t)v;                              // prime, mu = -1
-              } else {
-                const uint32_t av = (uint32_t)(v < 0 ? -v : v);
-                if (av <= p) continue;                        // mu(m) = 0 or lpf(m) <= p
-                m = 2 * (uint64_t)(cb - i) + 1;
-                sg = v > 0 ? 1 : -1;
-              }
-              const uint64_t n = cpx_div_leaf(xp, xpf, (uint32_t)m);   // lo_w <= n < hi_w
-              const uint32_t cn = count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
-              if (sg > 0) { acc[g] += cn; smu++; } else { acc[g] -= cn; smu--; }
-            }
-            // sum of mu over the wave (one candidate per lane at most: two ballots)
-            int32_t wm;
-            if (ib - ia <= cstep)
-              wm = (int32_t)__popcll(__ballot(smu > 0)) - (int32_t)__popcll(__ballot(smu < 0));
-            else
-              wm = (int32_t)wave_sum((uint32_t)smu);
-            if (lane == 0) sh.u.st.wm[it][wv] = wm;
+          if (f != f0) v = hard_fetch(bd, cb, sparse, i);
+          uint32_t m;
+          int32_t sg;
+          if (sparse) {
+            m = (uint32_t)v;                                  // prime, mu = -1
+            sg = -1;
+          } else {                                            // mu(m) = 0 or lpf(m) <= p: no leaf
+            const uint32_t av = (uint32_t)(v < 0 ? -v : v);
+            m = 2 * (cb - i) + 1;
+            sg = av <= p ? 0 : (v > 0 ? 1 : -1);
           }
-          const uint32_t nk = hard_next_leafy<T, K>(sh, it + 1);
-          if (nk < clen * G) {
-            uint32_t na, nb;
-            hard_wave_range<T, K>(bd, sh.gm[nk % G].y, sh, nk, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, na, nb);
-            if (na + rep * CPX_WAVE + lane < nb)
-              pf = hard_fetch(bd, sh.u.st.cbase[nk], sh.u.st.ccnt[nk] & HARD_SPARSE,
-                              na + rep * CPX_WAVE + lane);
+          int32_t val = 0;
+          if (sg) {
+            const uint64_t xp = sh.u.st.xp[it];
+            const uint64_t n = cpx_div_leaf(xp, (double)xp, m);   // lo_w <= n < hi_w
+            const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
+            val = sg > 0 ? cn : -cn;
+          }
+#pragma unroll
+          for (uint32_t h = 0; h < G; h++) {
+            acc[h] += g == h ? (int64_t)val : 0;
+            wm[h] += (int32_t)__popcll(__ballot(g == h && sg > 0)) - (int32_t)__popcll(__ballot(g == h && sg < 0));
+          }
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (uint32_t g = 0; g < G; g++)
+            if (g < gs) sh.u.st.wm[it0 + g][wv] = wm[g];
+        }
+        // next leafy stage
+        nk = hard_next_leafy<T, K>(sh, it0 + gs) >> lgs;
+        if (nk < clen) {
+          ntot = hard_stage_list<T, K>(bd, sh, nk, lgs, hard_stage_bits<T, K>(sh, nk, lgs), wv, lo_w,
+                                       hi_w, inv_lo_w, inv_hi_w, nia, npre);
+          if (f0 < ntot) {
+            uint32_t i;
+            const uint32_t it = (nk << lgs) + hard_list_entry(f0, nia, npre, i);
+            pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
           }
         }
       }
@@ -719,11 +780,11 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
         if (lane == 0) sh.u.st.dw[k][wv] = (uint16_t)wd;   // this wave's removals
       }
     }
-    hard_flush<T, K>(bd, sh, s0, clen, running, rep == 0, acc);
+    hard_flush<T, K>(bd, sh, s0, clen, gs, running, rep == 0, acc);
     s0 += clen;
-    clen = min((uint32_t)HARD_SST, nst - s0);
+    clen = min(sst, nst - s0);
     if (clen) {
-      hard_chunk_table<T, K>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
+      hard_chunk_table<T, K>(bd, c, s0, clen, lgs, lo, hi, inv_lo, inv_hi, sh);
       __syncthreads();
     }
   }
