@@ -95,7 +95,7 @@ __host__ __device__ __forceinline__ uint32_t cpx_fastmod32(uint32_t a, uint32_t 
 // segment once for up to CPX_GROUP queries of similar x (same wheel), and
 // evaluates every member's leaves, counters and P2 counts against it.
 #ifndef CPX_GROUP
-#define CPX_GROUP 4
+#define CPX_GROUP 8
 #endif
 struct GroupDev {
   uint32_t n;                   // members, sorted by x

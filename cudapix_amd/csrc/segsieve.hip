@@ -39,7 +39,7 @@
 #define P2_WAVE_P 1024     // P2 sieve: primes below this are crossed by a whole wave
 #endif
 static_assert(CPX_GROUP == 1 || CPX_GROUP == 2 || CPX_GROUP == 4 || CPX_GROUP == 8,
-              "a chunk's leafy mask word holds whole stages");
+              "member slots are powers of two, so a leafy-mask word holds whole stages");
 
 // A group member's share of the current segment (zero when its hard range
 // ends before the segment).
@@ -183,14 +183,14 @@ __device__ __forceinline__ int32_t hard_fetch(const BatchDev &bd, uint32_t cbase
 // sparse items (m = the i-th prime, n falls with i) it counts those with
 // n >= lo_w and is the wave's end.  Clears the per-wave sums of mu.  Caller
 // syncs before and after.
-template <int T, int K>
+template <int T, int K, int GS>
 __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c, uint32_t s0,
-                                                 uint32_t clen, uint32_t lgs, uint64_t lo, uint64_t hi,
+                                                 uint32_t clen, uint64_t lo, uint64_t hi,
                                                  double inv_lo, double inv_hi, SegShared<T, K> &sh)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
   constexpr uint64_t WB = (uint64_t)CPX_WAVE * 32 * K;
-  const uint32_t nit = clen << lgs;
+  const uint32_t nit = clen * GS;
   for (uint32_t k = threadIdx.x; k < nit * NW; k += T) (&sh.u.st.wm[0][0])[k] = 0;
   for (uint32_t k = threadIdx.x; k < clen; k += T) {
     const uint64_t p = bd.primes[c + s0 + k];
@@ -204,7 +204,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
     sh.u.st.oseg[k] = (uint32_t)((mm - start) >> 1);
   }
   for (uint32_t it = threadIdx.x; it < nit; it += T) {
-    const uint32_t k = it >> lgs, g = it & ((1u << lgs) - 1u);
+    const uint32_t k = it / GS, g = it % GS;
     const GroupMember &m = sh.gm[g];
     uint32_t ncand = 0, cbase = 0;
     bool sparse = false;
@@ -259,7 +259,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
       const uint64_t mtop = 2 * (uint64_t)cb + 1;            // m = mtop - 2i
       b = mtop > M ? (mtop - M + 1) / 2 : 0;
     } else {
-      const uint64_t y = sh.gm[it & ((1u << lgs) - 1u)].y;
+      const uint64_t y = sh.gm[it % GS].y;
       const uint64_t pm = pi_lookup(bd.pitab, M < y ? M : y);
       b = pm > cb ? pm - cb : 0;
     }
@@ -351,21 +351,21 @@ __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const Seg
 // members (lbits): member g's candidates [ia[g], ia[g] + n_g) are list
 // entries [pre[g], pre[g] + n_g), pre[] non-decreasing.  Returns the list
 // length; everything here is wave-uniform.
-template <int T, int K>
+template <int T, int K, int GS>
 __device__ __forceinline__ uint32_t hard_stage_list(const BatchDev &bd, const SegShared<T, K> &sh,
-                                                    uint32_t k, uint32_t lgs, uint32_t lbits,
+                                                    uint32_t k, uint32_t lbits,
                                                     uint32_t wv, uint64_t lo_w, uint64_t hi_w,
                                                     double inv_lo_w, double inv_hi_w,
-                                                    uint32_t (&ia)[CPX_GROUP], uint32_t (&pre)[CPX_GROUP])
+                                                    uint32_t (&ia)[GS], uint32_t (&pre)[GS])
 {
   uint32_t tot = 0;
 #pragma unroll
-  for (uint32_t g = 0; g < CPX_GROUP; g++) {
+  for (uint32_t g = 0; g < GS; g++) {
     pre[g] = tot;
     ia[g] = 0;
     if ((lbits >> g) & 1u) {
       uint32_t a, b;
-      hard_wave_range<T, K>(bd, sh.gm[g].y, sh, (k << lgs) + g, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
+      hard_wave_range<T, K>(bd, sh.gm[g].y, sh, k * GS + g, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
       a = (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
       b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
       if (b > a) { ia[g] = a; tot += b - a; }
@@ -375,43 +375,43 @@ __device__ __forceinline__ uint32_t hard_stage_list(const BatchDev &bd, const Se
 }
 
 // Member g and candidate index i of list entry f.
-__device__ __forceinline__ uint32_t hard_list_entry(uint32_t f, const uint32_t (&ia)[CPX_GROUP],
-                                                    const uint32_t (&pre)[CPX_GROUP], uint32_t &i)
+template <int GS>
+__device__ __forceinline__ uint32_t hard_list_entry(uint32_t f, const uint32_t (&ia)[GS],
+                                                    const uint32_t (&pre)[GS], uint32_t &i)
 {
   uint32_t g = 0;
 #pragma unroll
-  for (uint32_t h = 1; h < CPX_GROUP; h++) g += f >= pre[h] ? 1u : 0u;
+  for (uint32_t h = 1; h < GS; h++) g += f >= pre[h] ? 1u : 0u;
   uint32_t a = ia[0], b = 0;
 #pragma unroll
-  for (uint32_t h = 1; h < CPX_GROUP; h++)
+  for (uint32_t h = 1; h < GS; h++)
     if (g == h) { a = ia[h]; b = pre[h]; }
   i = a + (f - b);
   return g;
 }
 
 // Leafy-mask bits of chunk stage k (one per member slot).
-template <int T, int K>
-__device__ __forceinline__ uint32_t hard_stage_bits(const SegShared<T, K> &sh, uint32_t k, uint32_t lgs)
+template <int T, int K, int GS>
+__device__ __forceinline__ uint32_t hard_stage_bits(const SegShared<T, K> &sh, uint32_t k)
 {
-  const uint32_t it0 = k << lgs, gs = 1u << lgs;
-  return (sh.u.st.lmask[it0 >> 5] >> (it0 & 31)) & (gs == 32 ? ~0u : (1u << gs) - 1u);
+  const uint32_t it0 = k * GS;
+  return (sh.u.st.lmask[it0 >> 5] >> (it0 & 31)) & (GS == 32 ? ~0u : (1u << GS) - 1u);
 }
 
-// End of a chunk of stages [s0, s0 + n) (all threads call; `running` is kept
-// by wave 0): writes, for every member g and its active stages b, T_b =
+// End of a chunk of stages [s0, s0 + n) (all threads call and keep
+// `running`): writes, for every member g and its active stages b, T_b =
 // survivors of the segment before stage b (replica 0) and this replica's sum
 // of mu, and adds to acc[g] the part of every leaf's phi that the in-wave
 // count left out -- the survivors of the earlier waves at that stage:
 //   sum_k sum_w wm[k,g][w] * (A_w - sum_{j<k} sum_{w'<w} dw[j][w'])
 // with A_w the earlier waves' survivors at the chunk start.  Then resets the
 // chunk's leafy mask.
-template <int T, int K>
+template <int T, int K, int GS>
 __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &sh, uint32_t s0,
-                                           uint32_t n, uint32_t gs, uint32_t &running, bool write_T,
-                                           int64_t (&acc)[CPX_GROUP])
+                                           uint32_t n, uint32_t &running, bool write_T,
+                                           int64_t (&acc)[GS])
 {
   constexpr uint32_t NW = T / CPX_WAVE;
-  constexpr uint32_t G = CPX_GROUP;
   const uint32_t lane = lane_id(), wv = wave_id();
   __syncthreads();
   if (NW > 1) {
@@ -430,38 +430,36 @@ __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &
       const int64_t D = (int64_t)A - (int64_t)(carry + incl - Gk);
       if (k < n) {
 #pragma unroll
-        for (uint32_t g = 0; g < G; g++)
-          if (g < gs) acc[g] += (int64_t)sh.u.st.wm[k * gs + g][wv] * D;
+        for (uint32_t g = 0; g < GS; g++) acc[g] += (int64_t)sh.u.st.wm[k * GS + g][wv] * D;
       }
       carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
     }
   }
-  if (wv == 0) {                       // T_b and the sums of mu per stage
-    for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
-      const uint32_t k = k0 + lane;
-      uint32_t d = 0;
-      if (k < n) {
+  // T_b and the sums of mu per stage: wave w writes those of members
+  // w, w + NW, ...; every wave keeps `running`
+  for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
+    const uint32_t k = k0 + lane;
+    uint32_t d = 0;
+    if (k < n) {
 #pragma unroll
-        for (uint32_t w = 0; w < NW; w++) d += sh.u.st.dw[k][w];
-      }
-      const uint32_t incl = wave_incl_scan(d);
-      if (k < n) {
-        const uint32_t Tk = running - (incl - d);
-#pragma unroll
-        for (uint32_t g = 0; g < G; g++) {
-          const GroupMember &m = sh.gm[g];
-          if (g >= gs || s0 + k >= m.nst) continue;
-          int32_t ms = 0;
-#pragma unroll
-          for (uint32_t w = 0; w < NW; w++) ms += sh.u.st.wm[k * gs + g][w];
-          if (write_T) bd.hard_T[m.toff + s0 + k] = Tk;
-          bd.hard_M[m.moff + s0 + k] = ms;
-        }
-      }
-      running -= (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
+      for (uint32_t w = 0; w < NW; w++) d += sh.u.st.dw[k][w];
     }
+    const uint32_t incl = wave_incl_scan(d);
+    if (k < n) {
+      const uint32_t Tk = running - (incl - d);
+      for (uint32_t g = wv; g < GS; g += NW) {
+        const GroupMember &m = sh.gm[g];
+        if (s0 + k >= m.nst) continue;
+        int32_t ms = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) ms += sh.u.st.wm[k * GS + g][w];
+        if (write_T) bd.hard_T[m.toff + s0 + k] = Tk;
+        bd.hard_M[m.moff + s0 + k] = ms;
+      }
+    }
+    running -= (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
   }
-  __syncthreads();
+  // no wave reads the leafy mask after the first barrier
   if (threadIdx.x < HARD_SCH / 32) sh.u.st.lmask[threadIdx.x] = 0;
   __syncthreads();
 }
@@ -580,82 +578,34 @@ __device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const GroupDev &
   }
 }
 
-// One tile: segment `seg` of a group, replica `rep` of `nrep`.  The wheel
-// and the stage primes sieve the segment once; every member's leaves are
-// evaluated against that one bitmap.
-template <int T, int K>
-__device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &tile, uint32_t *lds,
-                                             SegShared<T, K> &sh)
+// The stage chunks of one tile with GS member slots (items k * GS + g):
+// leaves, cross-off and chunk flushes; leaves the members' sums of
+// mu * phi in sh.red_s.
+template <int T, int K, int GS>
+__device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> &sh, uint32_t *lds,
+                                            uint32_t c, uint32_t nst, uint32_t rep, uint32_t cstep,
+                                            uint64_t lo, uint64_t hi, uint64_t lo_w, uint32_t cnt,
+                                            uint32_t cnt_lo)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
   constexpr uint32_t BPT = 32 * K;
-  constexpr uint32_t WB = CPX_WAVE * BPT;                // bits per wave
-  constexpr uint32_t G = CPX_GROUP;
+  constexpr uint32_t WB = CPX_WAVE * BPT;
+  constexpr uint32_t SST = HARD_SST < HARD_SCH / GS ? HARD_SST : HARD_SCH / GS;   // stages per chunk
   const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
-  const uint64_t SB = (uint64_t)BPT * T;
-  const uint64_t seg = tile.seg;
-  const uint64_t lo = seg * 2 * SB;
-  const uint64_t hi = lo + 2 * SB;
-  const GroupDev &grp = bd.groups[tile.group];
-  const uint32_t rep = tile.rep;
-  const uint32_t c = bd.queries[grp.q[0]].c;           // members share the wheel
-  // member slots gs = 2^lgs >= members; a chunk holds HARD_SCH / gs stages
-  const uint32_t lgs = grp.n <= 1 ? 0u : (uint32_t)(32 - __builtin_clz(grp.n - 1));
-  const uint32_t gs = 1u << lgs;
-  const uint32_t sst = min((uint32_t)HARD_SST, (uint32_t)HARD_SCH >> lgs);
-  const WheelInfo wi = bd.winfo[c];
-  // members' rows of this segment: counters of the active stages only (the
-  // plan sized each row for ri.act stages)
-  if (t < G) {
-    GroupMember m{};
-    if (t < grp.n) {
-      const QueryDev &q = bd.queries[grp.q[t]];
-      if (seg < q.hard_nseg) {
-        const uint64_t row = q.hard_row0 + seg;
-        const HardRowInfo ri = bd.rowinfo[row];
-        m.y = q.y;
-        m.stage_off = q.stage_off;
-        m.toff = q.tm_off + ri.toff;
-        m.moff = q.m_off + ri.moff + (uint64_t)rep * ri.act;
-        m.acc_slot = q.mrow_base + ri.mrow + rep;
-        m.row = row;
-        m.nst = min(q.nstage, ri.act);
-        m.on = 1;
-        m.p2 = seg >= q.p2_seg0 && seg < q.p2_seg0 + q.p2_nseg;
-      }
-    }
-    sh.gm[t] = m;
-  }
-  // this wave's integers [lo_w, hi_w); this lane's first bit, for t*32K mod p
-  const uint64_t lo_w = lo + 2 * (uint64_t)wv * WB, hi_w = lo_w + 2 * (uint64_t)WB;
+  const uint64_t hi_w = lo_w + 2 * (uint64_t)WB;
   const double inv_lo_w = lo_w ? 1.0 / (double)lo_w : 0.0, inv_hi_w = 1.0 / (double)hi_w;
+  const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
   const uint32_t b0 = t * BPT;                          // < 2^20
   const float fb0 = (float)b0;
-  const uint32_t cstep = CPX_WAVE * (uint32_t)tile.nrep;   // candidate stride of a lane
-  const double inv_lo = lo ? 1.0 / (double)lo : 0.0, inv_hi = 1.0 / (double)hi;
-
-  seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
-  if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
-  __syncthreads();
-  uint32_t nst = 0, p2any = 0;                          // stages of the segment: the members' most
-#pragma unroll
-  for (uint32_t g = 0; g < G; g++) { nst = max(nst, sh.gm[g].nst); p2any |= sh.gm[g].p2; }
-  uint32_t cnt_lo = own_count_lo<K>(lds, t), cnt = cnt_lo;   // survivors: first half, all
-#pragma unroll 8
-  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
-  {
-    uint32_t ws = wave_sum(cnt);
-    if (lane == 0) sh.wsum[wv] = (int32_t)ws;
-  }
-  uint32_t s0 = 0, clen = min(sst, nst);
-  if (clen) hard_chunk_table<T, K>(bd, c, s0, clen, lgs, lo, hi, inv_lo, inv_hi, sh);
+  uint32_t s0 = 0, clen = min(SST, nst);
+  if (clen) hard_chunk_table<T, K, GS>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
   __syncthreads();
   uint32_t running = 0;                                 // survivors before the next stage
   for (uint32_t w = 0; w < NW; w++) running += (uint32_t)sh.wsum[w];
 
-  int64_t acc[G];
+  int64_t acc[GS];
 #pragma unroll
-  for (uint32_t g = 0; g < G; g++) acc[g] = 0;
+  for (uint32_t g = 0; g < GS; g++) acc[g] = 0;
   while (clen) {
     {
       const uint32_t ws = wave_sum(cnt);                // this wave's survivors at the chunk start
@@ -664,20 +614,20 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
     // one-ahead prefetch: the next leafy stage's list and this lane's first
     // candidate there
     const uint32_t f0 = rep * CPX_WAVE + lane;          // this lane's first list entry
-    uint32_t nk = hard_next_leafy<T, K>(sh, 0) >> lgs;  // next leafy stage (>= clen: none)
-    uint32_t nia[G], npre[G], ntot = 0;
+    uint32_t nk = hard_next_leafy<T, K>(sh, 0) / GS;   // next leafy stage (>= clen: none)
+    uint32_t nia[GS], npre[GS], ntot = 0;
     int32_t pf = 0;
     if (nk < clen) {
-      ntot = hard_stage_list<T, K>(bd, sh, nk, lgs, hard_stage_bits<T, K>(sh, nk, lgs), wv, lo_w, hi_w,
-                                   inv_lo_w, inv_hi_w, nia, npre);
+      ntot = hard_stage_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lo_w, hi_w,
+                                       inv_lo_w, inv_hi_w, nia, npre);
       if (f0 < ntot) {
         uint32_t i;
-        const uint32_t it = (nk << lgs) + hard_list_entry(f0, nia, npre, i);
+        const uint32_t it = nk * GS + hard_list_entry<GS>(f0, nia, npre, i);
         pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
       }
     }
     for (uint32_t k = 0; k < clen; k++) {
-      const uint32_t it0 = k << lgs;
+      const uint32_t it0 = k * GS;
       const uint32_t p = sh.u.st.p[k];
       if (k == nk) {
         // ---- leaves of p whose n falls in this wave's bits (state: crossed
@@ -687,13 +637,13 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
           const uint32_t incl = wave_incl_scan(cnt);
           sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
-        int32_t wm[G];                                  // sum of mu of the wave's leaves per member
+        int32_t wm[GS];                                 // sum of mu of the wave's leaves per member
 #pragma unroll
-        for (uint32_t g = 0; g < G; g++) wm[g] = 0;
+        for (uint32_t g = 0; g < GS; g++) wm[g] = 0;
         int32_t v = pf;
         for (uint32_t f = f0; f < ntot; f += cstep) {
           uint32_t i;
-          const uint32_t g = hard_list_entry(f, nia, npre, i);
+          const uint32_t g = hard_list_entry<GS>(f, nia, npre, i);
           const uint32_t it = it0 + g;
           const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
           const bool sparse = cc & HARD_SPARSE;
@@ -716,24 +666,23 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
             val = sg > 0 ? cn : -cn;
           }
 #pragma unroll
-          for (uint32_t h = 0; h < G; h++) {
+          for (uint32_t h = 0; h < GS; h++) {
             acc[h] += g == h ? (int64_t)val : 0;
             wm[h] += (int32_t)__popcll(__ballot(g == h && sg > 0)) - (int32_t)__popcll(__ballot(g == h && sg < 0));
           }
         }
         if (lane == 0) {
 #pragma unroll
-          for (uint32_t g = 0; g < G; g++)
-            if (g < gs) sh.u.st.wm[it0 + g][wv] = wm[g];
+          for (uint32_t g = 0; g < GS; g++) sh.u.st.wm[it0 + g][wv] = wm[g];
         }
         // next leafy stage
-        nk = hard_next_leafy<T, K>(sh, it0 + gs) >> lgs;
+        nk = hard_next_leafy<T, K>(sh, it0 + GS) / GS;
         if (nk < clen) {
-          ntot = hard_stage_list<T, K>(bd, sh, nk, lgs, hard_stage_bits<T, K>(sh, nk, lgs), wv, lo_w,
-                                       hi_w, inv_lo_w, inv_hi_w, nia, npre);
+          ntot = hard_stage_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lo_w,
+                                           hi_w, inv_lo_w, inv_hi_w, nia, npre);
           if (f0 < ntot) {
             uint32_t i;
-            const uint32_t it = (nk << lgs) + hard_list_entry(f0, nia, npre, i);
+            const uint32_t it = nk * GS + hard_list_entry<GS>(f0, nia, npre, i);
             pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
           }
         }
@@ -780,18 +729,90 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
         if (lane == 0) sh.u.st.dw[k][wv] = (uint16_t)wd;   // this wave's removals
       }
     }
-    hard_flush<T, K>(bd, sh, s0, clen, gs, running, rep == 0, acc);
+    hard_flush<T, K, GS>(bd, sh, s0, clen, running, rep == 0, acc);
     s0 += clen;
-    clen = min(sst, nst - s0);
+    clen = min(SST, nst - s0);
     if (clen) {
-      hard_chunk_table<T, K>(bd, c, s0, clen, lgs, lo, hi, inv_lo, inv_hi, sh);
+      hard_chunk_table<T, K, GS>(bd, c, s0, clen, lo, hi, inv_lo, inv_hi, sh);
       __syncthreads();
     }
   }
 #pragma unroll
-  for (uint32_t g = 0; g < G; g++) {
+  for (uint32_t g = 0; g < GS; g++) {
     const long long a = wave_sum((long long)acc[g]);
     if (lane == 0) sh.red_s[g][wv] = a;
+  }
+}
+
+// One tile: segment `seg` of a group, replica `rep` of `nrep`.  The wheel
+// and the stage primes sieve the segment once; every member's leaves are
+// evaluated against that one bitmap.
+template <int T, int K>
+__device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &tile, uint32_t *lds,
+                                             SegShared<T, K> &sh)
+{
+  constexpr uint32_t NW = T / CPX_WAVE;
+  constexpr uint32_t BPT = 32 * K;
+  constexpr uint32_t WB = CPX_WAVE * BPT;                // bits per wave
+  constexpr uint32_t G = CPX_GROUP;
+  const uint32_t t = threadIdx.x, lane = lane_id(), wv = wave_id();
+  const uint64_t SB = (uint64_t)BPT * T;
+  const uint64_t seg = tile.seg;
+  const uint64_t lo = seg * 2 * SB;
+  const uint64_t hi = lo + 2 * SB;
+  const GroupDev &grp = bd.groups[tile.group];
+  const uint32_t rep = tile.rep;
+  const uint32_t c = bd.queries[grp.q[0]].c;           // members share the wheel
+  const WheelInfo wi = bd.winfo[c];
+  // members' rows of this segment: counters of the active stages only (the
+  // plan sized each row for ri.act stages)
+  if (t < G) {
+    GroupMember m{};
+    if (t < grp.n) {
+      const QueryDev &q = bd.queries[grp.q[t]];
+      if (seg < q.hard_nseg) {
+        const uint64_t row = q.hard_row0 + seg;
+        const HardRowInfo ri = bd.rowinfo[row];
+        m.y = q.y;
+        m.stage_off = q.stage_off;
+        m.toff = q.tm_off + ri.toff;
+        m.moff = q.m_off + ri.moff + (uint64_t)rep * ri.act;
+        m.acc_slot = q.mrow_base + ri.mrow + rep;
+        m.row = row;
+        m.nst = min(q.nstage, ri.act);
+        m.on = 1;
+        m.p2 = seg >= q.p2_seg0 && seg < q.p2_seg0 + q.p2_nseg;
+      }
+    }
+    sh.gm[t] = m;
+  }
+  const uint64_t lo_w = lo + 2 * (uint64_t)wv * WB;     // this wave's integers from lo_w
+  const uint32_t cstep = CPX_WAVE * (uint32_t)tile.nrep;   // candidate stride of a lane
+
+  seg_fill_wheel(lds, K, T, lo, bd.wheel + wi.off, wi.period);
+  if (t < HARD_SCH / 32) sh.u.st.lmask[t] = 0;
+  __syncthreads();
+  uint32_t nst = 0, p2any = 0;                          // stages of the segment: the members' most
+#pragma unroll
+  for (uint32_t g = 0; g < G; g++) { nst = max(nst, sh.gm[g].nst); p2any |= sh.gm[g].p2; }
+  uint32_t cnt_lo = own_count_lo<K>(lds, t), cnt = cnt_lo;   // survivors: first half, all
+#pragma unroll 8
+  for (uint32_t j = K / 2; j < (uint32_t)K; j++) cnt += cpx_popc32(lds[own_addr(t, j, K)]);
+  {
+    uint32_t ws = wave_sum(cnt);
+    if (lane == 0) sh.wsum[wv] = (int32_t)ws;
+  }
+  switch (grp.n) {                                      // member slots: a power of two
+  case 1: hard_chunks<T, K, 1>(bd, sh, lds, c, nst, rep, cstep, lo, hi, lo_w, cnt, cnt_lo); break;
+#if CPX_GROUP >= 2
+  case 2: hard_chunks<T, K, 2>(bd, sh, lds, c, nst, rep, cstep, lo, hi, lo_w, cnt, cnt_lo); break;
+#endif
+#if CPX_GROUP >= 4
+  case 3: case 4: hard_chunks<T, K, 4>(bd, sh, lds, c, nst, rep, cstep, lo, hi, lo_w, cnt, cnt_lo); break;
+#endif
+#if CPX_GROUP >= 8
+  default: hard_chunks<T, K, 8>(bd, sh, lds, c, nst, rep, cstep, lo, hi, lo_w, cnt, cnt_lo); break;
+#endif
   }
   __syncthreads();
   if (t < G && sh.gm[t].on) {
