@@ -62,19 +62,47 @@ def primes_upto(n):
     return np.nonzero(s)[0]
 
 
+GROUP = 8   # queries per shared segment sieve (the library's group capacity)
+
+
+def sieve_groups(queries, primes):
+    """The library's query groups: increasing x, the same wheel c, at most
+    GROUP members, and every member's P2 start (isqrt(x) rounded down to a
+    multiple of 64) at or above every member's largest stage prime (the
+    largest prime <= x^(1/4))."""
+    import numpy as np
+    order = sorted((t for t in queries if t["x"] >= 16384), key=lambda t: t["x"])
+    groups, cur, smax, p2min = [], [], 0, None
+    for t in order:
+        qi = np.searchsorted(primes, iroot(t["x"], 4), side="right")
+        si = int(primes[qi - 1]) if qi > int(t["c"]) else 0
+        pi = iroot(t["x"], 2) & ~63
+        if cur and (len(cur) == GROUP or int(cur[0]["c"]) != int(t["c"])
+                    or max(smax, si) > min(p2min, pi)):
+            groups.append(cur)
+            cur, smax, p2min = [], 0, None
+        cur.append(t)
+        smax = max(smax, si)
+        p2min = pi if p2min is None else min(p2min, pi)
+    if cur:
+        groups.append(cur)
+    return groups
+
+
 def algorithmic_lds_bytes(queries, primes):
-    """Hard-leaf LDS work of the DR staged sieve, per query (x, y, c):
+    """Segment-sieve LDS work of the DR staged sieve, per group of queries
+    that share it (a single query is a group of one): with z and p_h the
+    largest of the members' z = x/y and floor(x^(1/4)),
     W = B_h * ceil(z/2)/8 + 8 * sum_{b=c+1}^{c+B_h} ceil((z/2)/p_b),
     one popcount/prefix pass over the odd-only bitmap per stage plus one
-    32-bit read-modify-write per cross-off; p_h = floor(x^(1/4))."""
+    32-bit read-modify-write per cross-off.  (Leaf evaluations are not in W:
+    they read a few words each.)"""
     import numpy as np
     tot = 0
-    for t in queries:
-        x, y, c = t["x"], t["y"], int(t["c"])
-        if x < 16384:
-            continue
-        z = x // y
-        ph = iroot(x, 4)
+    for g in sieve_groups(queries, primes):
+        c = int(g[0]["c"])
+        z = max(t["x"] // t["y"] for t in g)
+        ph = max(iroot(t["x"], 4) for t in g)
         ps = primes[c:np.searchsorted(primes, ph, side="right")]
         bh = len(ps)
         half = (z + 1) // 2
@@ -129,7 +157,8 @@ def cpu_baseline(xs, budget_s=20.0):
 
 def profile_run(args):
     """One workload only, for profiler runs: `single` = pi(x) on a warm
-    context (default shape), `batch` = the 512-query batched step, `copy` = a
+    context (default shape), `batch` = the 512-query batched step (with
+    --tuning as the timed run's autotune chose it), `copy` = a
     device-to-device copy of --copy-mib MiB (the byte-counter calibration)."""
     if args.profile == "copy":
         import torch
@@ -157,6 +186,9 @@ def profile_run(args):
               flush=True)
     else:
         xs = load_json("queries_c3.json")["x"]
+        if args.tuning:
+            t, k, a, c = args.tuning.split(",")
+            ctx.set_tuning(int(t), int(k), float(a), int(c))
         ref = ctx.pi_batch(xs)
         ms = []
         for _ in range(args.reps):
@@ -186,6 +218,8 @@ def main():
     ap.add_argument("--x", type=float, default=1e16, help="x of --profile single")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="bytes of --profile copy")
+    ap.add_argument("--tuning", default="",
+                    help="threads,kwords,alpha_scale,c for --profile batch (as the timed run's autotune chose)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and this "
@@ -320,7 +354,8 @@ def main():
         out["served"] = served
 
     # ---- roofline of the persistent segment kernel (the LDS bound of the
-    # staged sieve; HBM and the integer VALU rate beside it)
+    # staged sieve shared by each query group; HBM and the integer VALU rate
+    # beside it)
     lds_peak, int_peak = ctx.probe_peaks()
     ach = w_lds / seg_avg_s / 1e9
     traffic = profiled_traffic("batch")

@@ -93,6 +93,7 @@ EXPORTS = [
     "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
     "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave", "cpx_probe_peaks",
     "cpx_get_tuning",
+    "cpx_set_tuning",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -177,6 +178,8 @@ def lib():
         "cpx_probe_peaks": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
         "cpx_get_tuning": (ctypes.c_int, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32),
                                           P(ctypes.c_double), P(ctypes.c_uint32)]),
+        "cpx_set_tuning": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                          ctypes.c_uint32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -391,6 +394,11 @@ class Context:
         self._chk(self._L.cpx_get_tuning(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(a),
                                          ctypes.byref(c)), "get_tuning")
         return {"seg_threads": t.value, "seg_kwords": k.value, "alpha_scale": a.value, "c": c.value}
+
+    def set_tuning(self, seg_threads, seg_kwords, alpha_scale, c):
+        """Use a tuning (as returned by tuning()) without re-timing it."""
+        self._chk(self._L.cpx_set_tuning(self.h, int(seg_threads), int(seg_kwords), float(alpha_scale),
+                                         int(c)), "set_tuning")
 
     def request_bytes(self, x):
         return self._L.cpx_request_bytes(self.h, u128(x))

@@ -721,6 +721,23 @@ int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, ui
   return CPX_OK;
 }
 
+// Set the tuning the autotuner would pick (for replaying a tuned
+// configuration without re-timing it).
+int cpx_set_tuning(cpx_ctx *c, uint32_t threads, uint32_t kwords, double alpha_scale, uint32_t wheel_c)
+{
+  CTX_GUARD(c);
+  const SegShape s{(int)threads, (int)kwords};
+  if (!seg_shape_supported(s)) return ctx_fail(c, CPX_EINVAL, "unsupported segment shape");
+  if (!(alpha_scale > 0.0 && alpha_scale < 64.0)) return ctx_fail(c, CPX_EINVAL, "alpha_scale must be in (0, 64)");
+  if (wheel_c < 1 || wheel_c > 8) return ctx_fail(c, CPX_EINVAL, "c must be in 1..8");
+  c->shape = s;
+  c->alpha_scale = alpha_scale;
+  c->opts.c = wheel_c;
+  c->shape_tuned = true;
+  c->generation++;
+  return CPX_OK;
+}
+
 int cpx_get_tuning(cpx_ctx *c, uint32_t *threads, uint32_t *kwords, double *alpha_scale, uint32_t *wheel_c)
 {
   CTX_GUARD(c);

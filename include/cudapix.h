@@ -182,6 +182,11 @@ int cpx_autotune(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, uint32_t *threads, 
 /* The context's current tuning (any pointer may be NULL). */
 int cpx_get_tuning(cpx_ctx *ctx, uint32_t *threads, uint32_t *kwords, double *alpha_scale,
                    uint32_t *wheel_c);
+/* Set a tuning (a supported shape, alpha factor in (0, 64), c in 1..8), e.g.
+ * one cpx_autotune picked earlier, without re-timing.  No reference
+ * counterpart (the reference fixes alpha and c per build). */
+int cpx_set_tuning(cpx_ctx *ctx, uint32_t threads, uint32_t kwords, double alpha_scale,
+                   uint32_t wheel_c);
 
 /* ---- serving (continuous batching) --------------------------------------- */
 typedef struct {
