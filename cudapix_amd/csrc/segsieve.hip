@@ -7,18 +7,25 @@
 // the contiguous bits [t*32K, (t+1)*32K) and keeps their survivor count (and
 // that of its first K/2 words) in registers; wave w owns 64 threads' bits.
 //
+// A tile is one segment of a group of up to CPX_GROUP queries (similar x,
+// same wheel): the sieve state after stage b does not depend on x, so the
+// tile sieves the segment once and evaluates every member's leaves against
+// it.
+//
 // Hard leaves (replaces S2glob::S2ctl + S2hardHost, src/sieve/S2_hard_device.cu:264-355,
 // src/sieve/S2_hard_host.cu:115-191; DR split p <= x^(1/4), src/Deleglise-Rivat/S3.cu:8):
 //   The segment starts as the wheel of the first c primes.  Stages b (p = p_b,
-//   c < b <= B, while p^2 <= x/lo) run in chunks of HARD_SCH: the block first
-//   tabulates the chunk (p, x/p, first multiple, leaf candidate range of the
-//   segment and its split among the waves), then every wave walks the chunk's
-//   stages on its own, with no block barrier:
-//     L  leaves of p whose n = x/(p m) falls in the wave's bits: m odd with
-//        mu(m) != 0 and lpf(m) > p from the mu*lpf table (p^2 <= y) or m prime
-//        (p^2 > y); the wave publishes its in-wave prefix counts and counts
-//        phi_wave(n) = in-wave prefix of the owning lane + popcounts of its
-//        words below n; sum mu * phi_wave per thread, sum mu per wave;
+//   c < b <= B, while p^2 <= x/lo for some member) run in chunks of HARD_SCH
+//   (stage, member) items: the block first tabulates the chunk (p, first
+//   multiple; per item x/p, the leaf candidate range of the segment and its
+//   split among the waves), then every wave walks the chunk's stages on its
+//   own, with no block barrier:
+//     L  leaves of p whose n = x/(p m) falls in the wave's bits, all members'
+//        in one list: m odd with mu(m) != 0 and lpf(m) > p from the mu*lpf
+//        table (p^2 <= y) or m prime (p^2 > y); the wave publishes its
+//        in-wave prefix counts and counts phi_wave(n) = in-wave prefix of the
+//        owning lane + popcounts of its words below n; sum mu * phi_wave per
+//        thread and member, sum mu per wave and member;
 //     X  cross off the odd multiples of p in the thread's own bits, recording
 //        each wave's removals.
 //   At the end of a chunk the block adds the survivors of the earlier waves
@@ -28,9 +35,10 @@
 //   decomposition the reference does with scanVectorized/addMultiply,
 //   src/sieve/S2_hard_device.cu:357-375).
 //
-// P2: the segments holding x/p for the P2 primes p in (y, sqrt x] get, on
-// replica 0 of their tile, the rest of a prime sieve (primes above the last
-// stage up to sqrt(hi)) and the P2 counts; see p2_epilogue.
+// P2: the segments holding x/p for a member's P2 primes p in (y, sqrt x]
+// get, on replica 0 of their tile, the rest of a prime sieve (primes above
+// the last stage up to sqrt(hi)) and each such member's P2 counts; see
+// p2_epilogue.
 #include "cpx_device.hpp"
 
 #define HARD_SCH 128       // items (stage x member slot) per chunk
