@@ -72,6 +72,10 @@ struct SegShared {
   unsigned long long red_u[T / CPX_WAVE];
   long long red_s[CPX_GROUP][T / CPX_WAVE];
   GroupMember gm[CPX_GROUP];
+  // each wave's leaf list of its current stage: member g's entries start at
+  // list offset lpre[w][g], with candidate index lia[w][g]
+  __attribute__((aligned(16))) uint32_t lpre[T / CPX_WAVE][CPX_GROUP];
+  uint32_t lia[T / CPX_WAVE][CPX_GROUP];
   union {
     struct {                     // hard: chunk table and per-wave counters
       // per item i = k * gs + g (stage k of the chunk, member g; gs member
@@ -356,46 +360,66 @@ __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const Seg
 }
 
 // This wave's leaves of chunk stage k as one list over the stage's leafy
-// members (lbits): member g's candidates [ia[g], ia[g] + n_g) are list
-// entries [pre[g], pre[g] + n_g), pre[] non-decreasing.  Returns the list
-// length; everything here is wave-uniform.
+// members (lbits): lane g < GS finds member g's candidate range [a, b) and
+// takes list offsets [my_lo, my_hi) = [pre_g, pre_g + b - a); the offsets
+// and starts go to the wave's list slots.  Returns the list length
+// (wave-uniform).  All lanes call.
 template <int T, int K, int GS>
-__device__ __forceinline__ uint32_t hard_stage_list(const BatchDev &bd, const SegShared<T, K> &sh,
-                                                    uint32_t k, uint32_t lbits,
-                                                    uint32_t wv, uint64_t lo_w, uint64_t hi_w,
-                                                    double inv_lo_w, double inv_hi_w,
-                                                    uint32_t (&ia)[GS], uint32_t (&pre)[GS])
+__device__ __forceinline__ uint32_t hard_lane_list(const BatchDev &bd, SegShared<T, K> &sh, uint32_t k,
+                                                   uint32_t lbits, uint32_t wv, uint32_t lane,
+                                                   uint64_t lo_w, uint64_t hi_w, double inv_lo_w,
+                                                   double inv_hi_w, uint32_t &my_lo, uint32_t &my_hi)
 {
-  uint32_t tot = 0;
-#pragma unroll
-  for (uint32_t g = 0; g < GS; g++) {
-    pre[g] = tot;
-    ia[g] = 0;
-    if ((lbits >> g) & 1u) {
-      uint32_t a, b;
-      hard_wave_range<T, K>(bd, sh.gm[g].y, sh, k * GS + g, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
-      a = (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
-      b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
-      if (b > a) { ia[g] = a; tot += b - a; }
-    }
+  uint32_t a = 0, n = 0;
+  if (lane < GS && ((lbits >> lane) & 1u)) {
+    uint32_t b;
+    hard_wave_range<T, K>(bd, sh.gm[lane].y, sh, k * GS + lane, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
+    n = b > a ? b - a : 0u;
   }
-  return tot;
+  const uint32_t incl = wave_incl_scan(n);
+  my_lo = incl - n;
+  my_hi = incl;
+  if (lane < GS) {
+    sh.lpre[wv][lane] = my_lo;
+    sh.lia[wv][lane] = a;
+  }
+  return (uint32_t)__builtin_amdgcn_readlane((int)incl, GS - 1);
 }
 
-// Member g and candidate index i of list entry f.
-template <int GS>
-__device__ __forceinline__ uint32_t hard_list_entry(uint32_t f, const uint32_t (&ia)[GS],
-                                                    const uint32_t (&pre)[GS], uint32_t &i)
+// Member g and candidate index i of entry f of this wave's list.
+template <int T, int K, int GS>
+__device__ __forceinline__ uint32_t hard_lane_entry(const SegShared<T, K> &sh, uint32_t wv, uint32_t f,
+                                                    uint32_t &i)
 {
   uint32_t g = 0;
+  if (GS > 1) {
+    uint32_t pre[GS];
 #pragma unroll
-  for (uint32_t h = 1; h < GS; h++) g += f >= pre[h] ? 1u : 0u;
-  uint32_t a = ia[0], b = 0;
+    for (uint32_t h = 0; h < GS; h++) pre[h] = sh.lpre[wv][h];
 #pragma unroll
-  for (uint32_t h = 1; h < GS; h++)
-    if (g == h) { a = ia[h]; b = pre[h]; }
-  i = a + (f - b);
+    for (uint32_t h = 1; h < GS; h++) g += f >= pre[h] ? 1u : 0u;
+  }
+  i = sh.lia[wv][g] + (f - sh.lpre[wv][g]);
   return g;
+}
+
+// Add one window of list entries [fb, fb + 64) (lane l holds entry fb + l:
+// val = mu * phi_wave, sg = mu, zero past the list) to the sums of member
+// lanes: lane g < GS gains the sums over its entries [my_lo, my_hi).  One
+// wave scan of val * 256 + sg (|val| <= 2^16, so a window's sum fits 31
+// bits and its low byte is the sum of sg), read at each member's ends.
+__device__ __forceinline__ void hard_lane_accumulate(int32_t val, int32_t sg, uint32_t fb, uint32_t my_lo,
+                                                     uint32_t my_hi, int64_t &acc, int32_t &wmv)
+{
+  const uint32_t S = wave_incl_scan((uint32_t)(val * 256 + sg));
+  const int32_t lo = (int32_t)min(max((int64_t)my_lo - (int64_t)fb, (int64_t)0), (int64_t)CPX_WAVE);
+  const int32_t hi = (int32_t)min(max((int64_t)my_hi - (int64_t)fb, (int64_t)0), (int64_t)CPX_WAVE);
+  const uint32_t shi = (uint32_t)__builtin_amdgcn_ds_bpermute((hi > 0 ? hi - 1 : 0) * 4, (int)S);
+  const uint32_t slo = (uint32_t)__builtin_amdgcn_ds_bpermute((lo > 0 ? lo - 1 : 0) * 4, (int)S);
+  const int32_t d = (int32_t)((hi > 0 ? shi : 0u) - (lo > 0 ? slo : 0u));
+  const int32_t sgs = (int32_t)((uint32_t)d << 24) >> 24;
+  acc += (d - sgs) >> 8;
+  wmv += sgs;
 }
 
 // Leafy-mask bits of chunk stage k (one per member slot).
@@ -416,8 +440,7 @@ __device__ __forceinline__ uint32_t hard_stage_bits(const SegShared<T, K> &sh, u
 // chunk's leafy mask.
 template <int T, int K, int GS>
 __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &sh, uint32_t s0,
-                                           uint32_t n, uint32_t &running, bool write_T,
-                                           int64_t (&acc)[GS])
+                                           uint32_t n, uint32_t &running, bool write_T, int64_t &acc)
 {
   constexpr uint32_t NW = T / CPX_WAVE;
   const uint32_t lane = lane_id(), wv = wave_id();
@@ -426,21 +449,35 @@ __device__ __forceinline__ void hard_flush(const BatchDev &bd, SegShared<T, K> &
     uint32_t A = 0;
 #pragma unroll
     for (uint32_t w = 0; w < NW; w++) A += w < wv ? sh.u.st.c0[w] : 0u;
-    uint32_t carry = 0;
-    for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
-      const uint32_t k = k0 + lane;
+    if (GS == 1) {                     // sums per lane: lane k takes stage k
+      uint32_t carry = 0;
+      for (uint32_t k0 = 0; k0 < n; k0 += CPX_WAVE) {
+        const uint32_t k = k0 + lane;
+        uint32_t Gk = 0;
+        if (k < n) {
+#pragma unroll
+          for (uint32_t w = 0; w < NW; w++) Gk += w < wv ? (uint32_t)sh.u.st.dw[k][w] : 0u;
+        }
+        const uint32_t incl = wave_incl_scan(Gk);
+        if (k < n) acc += (int64_t)sh.u.st.wm[k][wv] * ((int64_t)A - (int64_t)(carry + incl - Gk));
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
+      }
+    } else {                           // sums in member lanes: the earlier waves' removals
+      // before each stage, R_k, go to this wave's part of the boundary table
+      // (no wave reads it after the first barrier; the next chunk rewrites it)
+      uint32_t *Rk = reinterpret_cast<uint32_t *>(&sh.u.st.wb[0][0]) + wv * CPX_WAVE;
+      const uint32_t k = lane;         // n <= 64 stages
       uint32_t Gk = 0;
       if (k < n) {
 #pragma unroll
         for (uint32_t w = 0; w < NW; w++) Gk += w < wv ? (uint32_t)sh.u.st.dw[k][w] : 0u;
       }
       const uint32_t incl = wave_incl_scan(Gk);
-      const int64_t D = (int64_t)A - (int64_t)(carry + incl - Gk);
-      if (k < n) {
-#pragma unroll
-        for (uint32_t g = 0; g < GS; g++) acc[g] += (int64_t)sh.u.st.wm[k * GS + g][wv] * D;
+      if (k < n) Rk[k] = incl - Gk;
+      if (lane < GS) {
+        for (uint32_t j = 0; j < n; j++)
+          acc += (int64_t)sh.u.st.wm[j * GS + lane][wv] * ((int64_t)A - (int64_t)Rk[j]);
       }
-      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, CPX_WAVE - 1);
     }
   }
   // T_b and the sums of mu per stage: wave w writes those of members
@@ -611,9 +648,8 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
   uint32_t running = 0;                                 // survivors before the next stage
   for (uint32_t w = 0; w < NW; w++) running += (uint32_t)sh.wsum[w];
 
-  int64_t acc[GS];
-#pragma unroll
-  for (uint32_t g = 0; g < GS; g++) acc[g] = 0;
+  // sums of mu * phi_wave: per lane (GS = 1), or member g's in lane g
+  int64_t acc = 0;
   while (clen) {
     {
       const uint32_t ws = wave_sum(cnt);                // this wave's survivors at the chunk start
@@ -623,14 +659,14 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
     // candidate there
     const uint32_t f0 = rep * CPX_WAVE + lane;          // this lane's first list entry
     uint32_t nk = hard_next_leafy<T, K>(sh, 0) / GS;   // next leafy stage (>= clen: none)
-    uint32_t nia[GS], npre[GS], ntot = 0;
+    uint32_t my_lo = 0, my_hi = 0, ntot = 0;
     int32_t pf = 0;
     if (nk < clen) {
-      ntot = hard_stage_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lo_w, hi_w,
-                                       inv_lo_w, inv_hi_w, nia, npre);
+      ntot = hard_lane_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lane, lo_w, hi_w,
+                                      inv_lo_w, inv_hi_w, my_lo, my_hi);
       if (f0 < ntot) {
         uint32_t i;
-        const uint32_t it = nk * GS + hard_list_entry<GS>(f0, nia, npre, i);
+        const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, i);
         pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
       }
     }
@@ -645,52 +681,49 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
           const uint32_t incl = wave_incl_scan(cnt);
           sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
-        int32_t wm[GS];                                 // sum of mu of the wave's leaves per member
-#pragma unroll
-        for (uint32_t g = 0; g < GS; g++) wm[g] = 0;
+        int32_t wmv = 0;                                // sum of mu: the wave's (GS = 1) or member lane's
         int32_t v = pf;
-        for (uint32_t f = f0; f < ntot; f += cstep) {
-          uint32_t i;
-          const uint32_t g = hard_list_entry<GS>(f, nia, npre, i);
-          const uint32_t it = it0 + g;
-          const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
-          const bool sparse = cc & HARD_SPARSE;
-          if (f != f0) v = hard_fetch(bd, cb, sparse, i);
-          uint32_t m;
-          int32_t sg;
-          if (sparse) {
-            m = (uint32_t)v;                                  // prime, mu = -1
-            sg = -1;
-          } else {                                            // mu(m) = 0 or lpf(m) <= p: no leaf
-            const uint32_t av = (uint32_t)(v < 0 ? -v : v);
-            m = 2 * (cb - i) + 1;
-            sg = av <= p ? 0 : (v > 0 ? 1 : -1);
+        for (uint32_t fb = rep * CPX_WAVE; fb < ntot; fb += cstep) {   // every lane runs every window
+          const uint32_t f = fb + lane;
+          int32_t val = 0, sg = 0;
+          if (f < ntot) {
+            uint32_t i;
+            const uint32_t it = it0 + hard_lane_entry<T, K, GS>(sh, wv, f, i);
+            const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
+            const bool sparse = cc & HARD_SPARSE;
+            if (fb != rep * CPX_WAVE) v = hard_fetch(bd, cb, sparse, i);
+            uint32_t m;
+            if (sparse) {
+              m = (uint32_t)v;                                // prime, mu = -1
+              sg = -1;
+            } else {                                          // mu(m) = 0 or lpf(m) <= p: no leaf
+              const uint32_t av = (uint32_t)(v < 0 ? -v : v);
+              m = 2 * (cb - i) + 1;
+              sg = av <= p ? 0 : (v > 0 ? 1 : -1);
+            }
+            if (sg) {
+              const uint64_t xp = sh.u.st.xp[it];
+              const uint64_t n = cpx_div_leaf(xp, (double)xp, m);   // lo_w <= n < hi_w
+              const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
+              val = sg > 0 ? cn : -cn;
+            }
           }
-          int32_t val = 0;
-          if (sg) {
-            const uint64_t xp = sh.u.st.xp[it];
-            const uint64_t n = cpx_div_leaf(xp, (double)xp, m);   // lo_w <= n < hi_w
-            const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
-            val = sg > 0 ? cn : -cn;
-          }
-#pragma unroll
-          for (uint32_t h = 0; h < GS; h++) {
-            acc[h] += g == h ? (int64_t)val : 0;
-            wm[h] += (int32_t)__popcll(__ballot(g == h && sg > 0)) - (int32_t)__popcll(__ballot(g == h && sg < 0));
+          if (GS == 1) {
+            acc += val;
+            wmv += (int32_t)__popcll(__ballot(sg > 0)) - (int32_t)__popcll(__ballot(sg < 0));
+          } else {
+            hard_lane_accumulate(val, sg, fb, my_lo, my_hi, acc, wmv);
           }
         }
-        if (lane == 0) {
-#pragma unroll
-          for (uint32_t g = 0; g < GS; g++) sh.u.st.wm[it0 + g][wv] = wm[g];
-        }
+        if (lane < GS) sh.u.st.wm[it0 + lane][wv] = wmv;
         // next leafy stage
         nk = hard_next_leafy<T, K>(sh, it0 + GS) / GS;
         if (nk < clen) {
-          ntot = hard_stage_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lo_w,
-                                           hi_w, inv_lo_w, inv_hi_w, nia, npre);
+          ntot = hard_lane_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lane, lo_w,
+                                          hi_w, inv_lo_w, inv_hi_w, my_lo, my_hi);
           if (f0 < ntot) {
             uint32_t i;
-            const uint32_t it = nk * GS + hard_list_entry<GS>(f0, nia, npre, i);
+            const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, i);
             pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
           }
         }
@@ -745,10 +778,11 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
       __syncthreads();
     }
   }
-#pragma unroll
-  for (uint32_t g = 0; g < GS; g++) {
-    const long long a = wave_sum((long long)acc[g]);
-    if (lane == 0) sh.red_s[g][wv] = a;
+  if (GS == 1) {
+    const long long a = wave_sum((long long)acc);
+    if (lane == 0) sh.red_s[0][wv] = a;
+  } else if (lane < GS) {
+    sh.red_s[lane][wv] = acc;
   }
 }
 
