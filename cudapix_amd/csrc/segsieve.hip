@@ -368,8 +368,17 @@ template <int T, int K, int GS>
 __device__ __forceinline__ uint32_t hard_lane_list(const BatchDev &bd, SegShared<T, K> &sh, uint32_t k,
                                                    uint32_t lbits, uint32_t wv, uint32_t lane,
                                                    uint64_t lo_w, uint64_t hi_w, double inv_lo_w,
-                                                   double inv_hi_w, uint32_t &my_lo, uint32_t &my_hi)
+                                                   double inv_hi_w, uint32_t &my_lo, uint32_t &my_hi,
+                                                   uint32_t &ia1)
 {
+  if (GS == 1) {                       // one member: every lane derives the range, kept in ia1
+    uint32_t a = 0, b = 0;
+    if (lbits & 1u) hard_wave_range<T, K>(bd, sh.gm[0].y, sh, k, wv, lo_w, hi_w, inv_lo_w, inv_hi_w, a, b);
+    ia1 = a;
+    my_lo = 0;
+    my_hi = b > a ? b - a : 0u;
+    return my_hi;
+  }
   uint32_t a = 0, n = 0;
   if (lane < GS && ((lbits >> lane) & 1u)) {
     uint32_t b;
@@ -386,11 +395,16 @@ __device__ __forceinline__ uint32_t hard_lane_list(const BatchDev &bd, SegShared
   return (uint32_t)__builtin_amdgcn_readlane((int)incl, GS - 1);
 }
 
-// Member g and candidate index i of entry f of this wave's list.
+// Member g and candidate index i of entry f of this wave's list (ia1: the
+// candidate start of a one-member list).
 template <int T, int K, int GS>
 __device__ __forceinline__ uint32_t hard_lane_entry(const SegShared<T, K> &sh, uint32_t wv, uint32_t f,
-                                                    uint32_t &i)
+                                                    uint32_t ia1, uint32_t &i)
 {
+  if (GS == 1) {
+    i = ia1 + f;
+    return 0;
+  }
   uint32_t g = 0;
   if (GS > 1) {
     uint32_t pre[GS];
@@ -571,6 +585,9 @@ __device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const GroupDev &
     if (!sh.gm[g].p2) continue;
     const QueryDev &q = bd.queries[grp.q[g]];
     const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    // x / p < z < 2^50 for every P2 prime: the reciprocal divide when p < 2^32
+    const bool fastdiv = q.sqrtx < (1ull << 32);
+    const double xf = (double)x;
     // survivors at or below p2_e (1, the stage primes and what is below
     // sqrt x are not counted): only the first segment of the range has any
     const uint32_t ce = q.p2_e >= lo ? count_below<T, K>(lds, sh, wp, (uint32_t)min((q.p2_e - lo + 1) >> 1, SB)) : 0u;
@@ -600,7 +617,8 @@ __device__ __forceinline__ void p2_epilogue(const BatchDev &bd, const GroupDev &
           const uint32_t i = __builtin_ctz(b);
           b &= b - 1;
           const uint64_t p = base + 2 * i + 1;
-          const uint64_t v = cpx_div128_to64(x, p);       // lo <= v < hi
+          const uint64_t v = fastdiv ? cpx_div_leaf(q.x_lo, xf, (uint32_t)p)
+                                     : cpx_div128_to64(x, p);   // lo <= v < hi
           loc += (unsigned long long)(count_below<T, K>(lds, sh, wp, (uint32_t)((v - lo + 1) >> 1)) - ce);
           np++;
         }
@@ -659,14 +677,14 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
     // candidate there
     const uint32_t f0 = rep * CPX_WAVE + lane;          // this lane's first list entry
     uint32_t nk = hard_next_leafy<T, K>(sh, 0) / GS;   // next leafy stage (>= clen: none)
-    uint32_t my_lo = 0, my_hi = 0, ntot = 0;
+    uint32_t my_lo = 0, my_hi = 0, ntot = 0, ia1 = 0;
     int32_t pf = 0;
     if (nk < clen) {
       ntot = hard_lane_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lane, lo_w, hi_w,
-                                      inv_lo_w, inv_hi_w, my_lo, my_hi);
+                                      inv_lo_w, inv_hi_w, my_lo, my_hi, ia1);
       if (f0 < ntot) {
         uint32_t i;
-        const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, i);
+        const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, ia1, i);
         pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
       }
     }
@@ -688,7 +706,7 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
           int32_t val = 0, sg = 0;
           if (f < ntot) {
             uint32_t i;
-            const uint32_t it = it0 + hard_lane_entry<T, K, GS>(sh, wv, f, i);
+            const uint32_t it = it0 + hard_lane_entry<T, K, GS>(sh, wv, f, ia1, i);
             const uint32_t cc = sh.u.st.ccnt[it], cb = sh.u.st.cbase[it];
             const bool sparse = cc & HARD_SPARSE;
             if (fb != rep * CPX_WAVE) v = hard_fetch(bd, cb, sparse, i);
@@ -720,10 +738,10 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
         nk = hard_next_leafy<T, K>(sh, it0 + GS) / GS;
         if (nk < clen) {
           ntot = hard_lane_list<T, K, GS>(bd, sh, nk, hard_stage_bits<T, K, GS>(sh, nk), wv, lane, lo_w,
-                                          hi_w, inv_lo_w, inv_hi_w, my_lo, my_hi);
+                                          hi_w, inv_lo_w, inv_hi_w, my_lo, my_hi, ia1);
           if (f0 < ntot) {
             uint32_t i;
-            const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, i);
+            const uint32_t it = nk * GS + hard_lane_entry<T, K, GS>(sh, wv, f0, ia1, i);
             pf = hard_fetch(bd, sh.u.st.cbase[it], sh.u.st.ccnt[it] & HARD_SPARSE, i);
           }
         }
