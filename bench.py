@@ -67,9 +67,9 @@ GROUP = 8   # queries per shared segment sieve (the library's group capacity)
 
 def sieve_groups(queries, primes):
     """The library's query groups: increasing x, the same wheel c, at most
-    GROUP members, and every member's P2 start (isqrt(x) rounded down to a
+    GROUP members, every member's P2 start (isqrt(x) rounded down to a
     multiple of 64) at or above every member's largest stage prime (the
-    largest prime <= x^(1/4))."""
+    largest prime <= x^(1/4)), and a query with y >= 2^28 alone."""
     import numpy as np
     order = sorted((t for t in queries if t["x"] >= 16384), key=lambda t: t["x"])
     groups, cur, smax, p2min = [], [], 0, None
@@ -77,8 +77,9 @@ def sieve_groups(queries, primes):
         qi = np.searchsorted(primes, iroot(t["x"], 4), side="right")
         si = int(primes[qi - 1]) if qi > int(t["c"]) else 0
         pi = iroot(t["x"], 2) & ~63
-        if cur and (len(cur) == GROUP or int(cur[0]["c"]) != int(t["c"])
-                    or max(smax, si) > min(p2min, pi)):
+        alone = t["y"] >= 1 << 28
+        if cur and (len(cur) == GROUP or int(cur[0]["c"]) != int(t["c"]) or alone
+                    or cur[-1]["y"] >= 1 << 28 or max(smax, si) > min(p2min, pi)):
             groups.append(cur)
             cur, smax, p2min = [], 0, None
         cur.append(t)

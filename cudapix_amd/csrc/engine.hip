@@ -19,6 +19,9 @@
 #include <cstring>
 #include <numeric>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 static double now_ms()
@@ -448,7 +451,9 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
 // in increasing x, with the same wheel, at most CPX_GROUP, and every
 // member's P2 range (from p2_e) above every member's stage primes (segment 0
 // loses its stage primes to the sieve, so they must not be counted as P2
-// primes of another member).
+// primes of another member).  A query with y >= 2^28 stays alone, which
+// keeps a wave's leaf list of a group (at most y/2 candidates per member)
+// below 2^30 entries.
 static void make_groups(const cpx_ctx *c, const std::vector<HostQuery> &hq,
                         const std::vector<QPlan> &qps, std::vector<GroupDev> &groups)
 {
@@ -457,12 +462,14 @@ static void make_groups(const cpx_ctx *c, const std::vector<HostQuery> &hq,
     if (qps[i].d.hard_nseg) hs.push_back(i);
   std::stable_sort(hs.begin(), hs.end(), [&](uint32_t a, uint32_t b) { return hq[a].x < hq[b].x; });
   GroupDev cur{};
+  bool cur_alone = false;
   uint64_t smax = 0, p2min = ~0ull;
   for (uint32_t i : hs) {
     const QueryDev &d = qps[i].d;
     const uint64_t si = d.nstage ? c->h_primes[d.c + d.nstage - 1] : 0;
     const uint64_t pi = d.p2_nseg ? d.p2_e : ~0ull;
-    if (cur.n && (cur.n == CPX_GROUP || qps[cur.q[0]].d.c != d.c ||
+    const bool alone = d.y >= (1ull << 28);
+    if (cur.n && (cur.n == CPX_GROUP || qps[cur.q[0]].d.c != d.c || alone || cur_alone ||
                   std::max(smax, si) > std::min(p2min, pi))) {
       groups.push_back(cur);
       cur = GroupDev{};
@@ -470,6 +477,7 @@ static void make_groups(const cpx_ctx *c, const std::vector<HostQuery> &hq,
       p2min = ~0ull;
     }
     cur.q[cur.n++] = i;
+    cur_alone = alone;
     smax = std::max(smax, si);
     p2min = std::min(p2min, pi);
   }
@@ -538,24 +546,101 @@ static void plan_group(const cpx_ctx *c, const GroupDev &g, uint32_t gi, std::ve
   }
 }
 
-// Run f(k) for k in [0, n) on up to `threads` host threads (inline for one).
+// Host worker pool for the planner: threads are started once per process
+// and woken per job (a planner call costs microseconds, so spawning
+// threads per call would dominate it).  run(n, threads, f) runs f(k) for k
+// in [0, n) on the caller and up to threads - 1 workers.  Jobs are
+// serialised by a mutex: contexts on several host threads plan one at a time.
+class PlanPool {
+public:
+  static PlanPool &get()
+  {
+    static PlanPool pool;
+    return pool;
+  }
+  template <typename F>
+  void run(size_t n, unsigned threads, F &f)
+  {
+    if (n == 0) return;
+    threads = (unsigned)std::min<size_t>(std::max(1u, threads), n);
+    threads = std::min<unsigned>(threads, (unsigned)workers_.size() + 1);
+    if (threads == 1) {
+      for (size_t k = 0; k < n; k++) f(k);
+      return;
+    }
+    std::lock_guard<std::mutex> job_lock(job_mu_);
+    std::atomic<size_t> next{0};
+    auto body = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+    };
+    std::function<void()> job = body;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &job;
+      want_ = threads - 1;
+      done_ = 0;
+      gen_++;
+    }
+    cv_.notify_all();
+    body();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return done_ == want_; });
+    job_ = nullptr;
+  }
+
+private:
+  PlanPool()
+  {
+    unsigned hc = std::thread::hardware_concurrency();
+    unsigned nw = std::min<unsigned>(hc ? hc : 4, 16);
+    for (unsigned i = 1; i < nw; i++) workers_.emplace_back([this] { loop(); });
+  }
+  ~PlanPool()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+      gen_++;
+    }
+    cv_.notify_all();
+    for (auto &w : workers_) w.join();
+  }
+  void loop()
+  {
+    uint64_t seen = 0;
+    for (;;) {
+      std::function<void()> *job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        if (taken_gen_ != gen_) { taken_gen_ = gen_; taken_ = 0; }
+        if (taken_ >= want_) continue;     // enough workers joined this job
+        taken_++;
+        job = job_;
+      }
+      (*job)();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_++;
+      }
+      done_cv_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex job_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  std::function<void()> *job_ = nullptr;
+  uint64_t gen_ = 0, taken_gen_ = 0;
+  unsigned want_ = 0, taken_ = 0, done_ = 0;
+  bool stop_ = false;
+};
+
 template <typename F>
 static void parallel_for(size_t n, unsigned threads, F f)
 {
-  if (n == 0) return;
-  threads = (unsigned)std::min<size_t>(std::max(1u, threads), n);
-  if (threads == 1) {
-    for (size_t k = 0; k < n; k++) f(k);
-    return;
-  }
-  std::atomic<size_t> next{0};
-  auto work = [&] {
-    for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
-  };
-  std::vector<std::thread> pool;
-  for (unsigned t = 1; t < threads; t++) pool.emplace_back(work);
-  work();
-  for (auto &th : pool) th.join();
+  PlanPool::get().run(n, threads, f);
 }
 
 static unsigned plan_threads(size_t nq)

@@ -426,8 +426,10 @@ __device__ __forceinline__ void hard_lane_accumulate(int32_t val, int32_t sg, ui
                                                      uint32_t my_hi, int64_t &acc, int32_t &wmv)
 {
   const uint32_t S = wave_incl_scan((uint32_t)(val * 256 + sg));
-  const int32_t lo = (int32_t)min(max((int64_t)my_lo - (int64_t)fb, (int64_t)0), (int64_t)CPX_WAVE);
-  const int32_t hi = (int32_t)min(max((int64_t)my_hi - (int64_t)fb, (int64_t)0), (int64_t)CPX_WAVE);
+  // list offsets are below 2^30: an item has at most y/2 candidates, and
+  // groups of more than one member have y < 2^28
+  const int32_t lo = min(max((int32_t)my_lo - (int32_t)fb, 0), CPX_WAVE);
+  const int32_t hi = min(max((int32_t)my_hi - (int32_t)fb, 0), CPX_WAVE);
   const uint32_t shi = (uint32_t)__builtin_amdgcn_ds_bpermute((hi > 0 ? hi - 1 : 0) * 4, (int)S);
   const uint32_t slo = (uint32_t)__builtin_amdgcn_ds_bpermute((lo > 0 ? lo - 1 : 0) * 4, (int)S);
   const int32_t d = (int32_t)((hi > 0 ? shi : 0u) - (lo > 0 ? slo : 0u));
@@ -722,7 +724,8 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
             if (sg) {
               const uint64_t xp = sh.u.st.xp[it];
               const uint64_t n = cpx_div_leaf(xp, (double)xp, m);   // lo_w <= n < hi_w
-              const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv, (uint32_t)((n - lo_w + 1) >> 1));
+              const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv,
+                                                              ((uint32_t)n - (uint32_t)lo_w + 1u) >> 1);
               val = sg > 0 ? cn : -cn;
             }
           }
