@@ -111,6 +111,21 @@ def algorithmic_lds_bytes(queries, primes):
     return tot
 
 
+def profiled_valu(workload):
+    """Measured VALU issue fraction of k_segments (SQ_INSTS_VALU over the
+    measured int32 issue peak) from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_segments_%s_pmc.md" % workload)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("| VALU issue rate |"):
+                    pct = line.split("=")[1].split("%")[0]
+                    return round(float(pct) / 100.0, 4)
+    except (OSError, IndexError, ValueError):
+        pass
+    return None
+
+
 def profiled_traffic(workload):
     """HBM bytes per k_segments launch from the committed rocprofv3 --pmc
     summary of the same workload (FETCH_SIZE and WRITE_SIZE, scaled by the
@@ -371,7 +386,10 @@ def main():
                            "hbm": {"achieved": round(traffic / seg_avg_s / 1e9, 1) if traffic else None,
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": round(traffic / seg_avg_s / 1e9 / HBM_PEAK_GBS, 5) if traffic else None},
-                           "int32_valu": {"peak": round(int_peak / 1e12, 2), "unit": "Tops/s (lane ops)"}}}
+                           "int32_valu": {"peak": round(int_peak / 1e12, 2), "unit": "Tops/s (lane ops)",
+                                          "issue_frac": profiled_valu("batch"),
+                                          "issue_frac_source": "SQ_INSTS_VALU / kernel time over the "
+                                                               "measured issue peak, profiles/k_segments_batch_pmc.md"}}}
     if "single_1e16" in out:
         s16 = out["single_1e16"]
         s16["lds_roofline_frac"] = round(s16["lds_algorithmic_GB"] / (s16["segments_ms"] / 1000.0)
