@@ -50,6 +50,19 @@ __device__ __forceinline__ uint64_t cpx_div_leaf(uint64_t a, double af, uint32_t
   return q;
 }
 
+// cpx_div_leaf with r = 1/m rounded to nearest given (a table of prime
+// reciprocals): af * r is then within a/m * 2^-52 (< 0.25) of a/m, and the
+// same rounding and remainder test apply.
+__device__ __forceinline__ uint64_t cpx_div_leaf_r(uint64_t a, double af, uint32_t m, double r)
+{
+  const double est = __builtin_fma(af, r, 0x1p52);
+  uint64_t q = (uint64_t)__builtin_bit_cast(long long, est) & ((1ull << 52) - 1);
+  const int64_t rem = (int64_t)(a - q * (uint64_t)m);
+  if (rem < 0) q--;
+  else if (rem >= (int64_t)m) q++;
+  return q;
+}
+
 // Sum of 32-bit values over the wavefront, in every lane (DPP scan, then lane 63).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {

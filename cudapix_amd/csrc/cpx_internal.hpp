@@ -147,6 +147,7 @@ struct BatchDev {
   // shared resident tables
   const PiEntry *pitab;  uint64_t pitab_n;
   const uint32_t *primes; uint64_t nprimes;
+  const double *prinv;                        // 1/p (nearest) of primes[i]
   const int32_t *mlpf;   uint64_t mlpf_n;     // mu(m)*lpf(m) for odd m, index m>>1
   const PiEntry *wheel;  const WheelInfo *winfo;
   // plan
@@ -183,6 +184,8 @@ hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s);
 hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sprimes,
                                  uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
                                  hipStream_t s);
+int easy_blocks_per_cu();
+hipError_t launch_recip(const uint32_t *primes, double *rinv, uint64_t n, hipStream_t s);
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes, uint64_t cap,
                                  hipStream_t s);
 hipError_t launch_mlpf(int32_t *mlpf, uint64_t nodd, const uint32_t *primes, uint32_t nsp,

@@ -77,7 +77,7 @@ static void dbuf_free(DBuf<T> &b)
 
 void free_tables(cpx_ctx *c)
 {
-  dbuf_free(c->pitab); dbuf_free(c->primes); dbuf_free(c->mlpf);
+  dbuf_free(c->pitab); dbuf_free(c->primes); dbuf_free(c->prinv); dbuf_free(c->mlpf);
   dbuf_free(c->scan_tot); dbuf_free(c->scan_base);
   c->pitab_nent = c->pitab_limit = c->prime_limit = c->nprimes = c->mlpf_limit = 0;
   c->h_primes.clear();
@@ -264,6 +264,8 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
   uint64_t np = 1 + e.count + (uint64_t)__builtin_popcount(e.bits & (uint32_t)((1ull << kk) - 1));
   if ((rc = dbuf_reserve(c, c->primes, np + 1, &c->table_bytes))) return rc;
   HIPCHK(launch_extract_primes(c->pitab.p, Pm, c->primes.p, np + 1, c->stream));
+  if ((rc = dbuf_reserve(c, c->prinv, np + 1, &c->table_bytes))) return rc;
+  HIPCHK(launch_recip(c->primes.p, c->prinv.p, np, c->stream));
   c->h_primes.resize(np);
   HIPCHK(hipMemcpyAsync(c->h_primes.data(), c->primes.p, np * sizeof(uint32_t),
                         hipMemcpyDeviceToHost, c->stream));
@@ -828,7 +830,7 @@ static BatchDev make_batchdev(cpx_ctx *c)
   char *P = (char *)c->d_plan, *O = (char *)c->d_out;
   BatchDev b{};
   b.pitab = c->pitab.p; b.pitab_n = c->pitab_nent;
-  b.primes = c->primes.p; b.nprimes = c->nprimes;
+  b.primes = c->primes.p; b.nprimes = c->nprimes; b.prinv = c->prinv.p;
   b.mlpf = c->mlpf.p; b.mlpf_n = c->mlpf.cap;
   b.wheel = c->wheel.p; b.winfo = c->winfo.p;
   uint32_t *cnt = (uint32_t *)(P + L.counters);
@@ -909,6 +911,15 @@ static int seg_grid(cpx_ctx *c)
 
 static const uint32_t GRID_Q = 256, GRID_T = 2048;
 
+// The easy-leaf kernel's grid: one resident round of blocks (a second,
+// partial round would leave most of the chip idle at its end).
+static uint32_t easy_grid(cpx_ctx *c)
+{
+  static int nb = -1;
+  if (nb < 0) nb = easy_blocks_per_cu();
+  return nb > 0 ? (uint32_t)(nb * c->cu_count) : GRID_T;
+}
+
 // The launch sequence in three parts; events between the parts time the
 // persistent segment kernel on its own stream (also when replayed as graphs).
 //   part 0: clear results + queue head, setup, ordinary, trivial, easy
@@ -922,7 +933,7 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
     HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
     HIPCHK(launch_ordinary(b, GRID_T, c->stream));
     HIPCHK(launch_trivial(b, GRID_T, c->stream));
-    HIPCHK(launch_easy(b, GRID_T, c->stream));
+    HIPCHK(launch_easy(b, easy_grid(c), c->stream));
   } else if (part == 1) {
     HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
   } else {
@@ -939,8 +950,9 @@ static int ensure_graph(cpx_ctx *c, const BatchDev &b, uint32_t nres_words)
     return CPX_OK;
   for (auto &g : c->gexec)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-  // kernel attributes and the occupancy query must happen outside capture
+  // kernel attributes and the occupancy queries must happen outside capture
   (void)seg_grid(c);
+  (void)easy_grid(c);
   for (int part = 0; part < 3; part++) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

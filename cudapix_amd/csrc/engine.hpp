@@ -59,6 +59,7 @@ struct cpx_ctx {
   DBuf<PiEntry> pitab;
   uint64_t pitab_nent = 0, pitab_limit = 0;     // pi(n) valid for n <= pitab_limit
   DBuf<uint32_t> primes;
+  DBuf<double> prinv;               // 1/p of primes[i], rounded to nearest
   uint64_t prime_limit = 0, nprimes = 0;
   std::vector<uint32_t> h_primes;
   DBuf<int32_t> mlpf;

@@ -185,6 +185,22 @@ __global__ __launch_bounds__(256) void k_extract(const PiEntry *__restrict__ tab
   }
 }
 
+// 1/p rounded to nearest (IEEE division) for every listed prime.
+__global__ __launch_bounds__(256) void k_recip(const uint32_t *__restrict__ primes, double *rinv, uint64_t n)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    rinv[i] = 1.0 / (double)primes[i];
+}
+
+hipError_t launch_recip(const uint32_t *primes, double *rinv, uint64_t n, hipStream_t s)
+{
+  if (!n) return hipSuccess;
+  uint64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_recip, dim3((uint32_t)g), dim3(256), 0, s, primes, rinv, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes,
                                  uint64_t cap, hipStream_t s)
 {

@@ -165,24 +165,35 @@ __global__ __launch_bounds__(256) void k_easy(BatchDev bd)
         const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
         if (k1 <= j + 1) continue;
         // four independent prime -> quotient -> pi-table chains per thread
-        // in flight; x/(p q) < x/p^2 < 2^50, so the reciprocal divide is exact
+        // in flight, without branches (a chain past k1 repeats index k and
+        // is masked); x/(p q) < x/p^2 < 2^50, so the reciprocal divide with
+        // the tabulated 1/q is exact
         const double xpf = (double)xp;
         uint64_t s = 0;
         for (uint64_t k = j + 1 + t; k < k1; k += 4 * B) {
           uint32_t qv[4];
+          double rv[4];
+          bool ok[4];
 #pragma unroll
-          for (int u = 0; u < 4; u++) qv[u] = k + u * B < k1 ? bd.primes[k + u * B] : 0u;
+          for (int u = 0; u < 4; u++) {
+            ok[u] = k + u * B < k1;
+            const uint64_t kk = ok[u] ? k + u * B : k;
+            qv[u] = bd.primes[kk];
+            rv[u] = bd.prinv[kk];
+          }
           PiEntry pe[4];
           uint32_t ix[4];
 #pragma unroll
           for (int u = 0; u < 4; u++) {
-            const uint64_t v = qv[u] ? cpx_div_leaf(xp, xpf, qv[u]) : 0;
+            const uint64_t v = cpx_div_leaf_r(xp, xpf, qv[u], rv[u]);
             ix[u] = (uint32_t)(((v & 63) + 1) >> 1) | (v >= 2 ? 0x80u : 0u);
             pe[u] = bd.pitab[v >> 6];
           }
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            if (qv[u]) s += (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
+          for (int u = 0; u < 4; u++) {
+            const uint32_t c = (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
+            s += ok[u] ? c : 0u;
+          }
         }
         sum += (i128)s;
       }
@@ -191,6 +202,13 @@ __global__ __launch_bounds__(256) void k_easy(BatchDev bd)
     i128 tt = block_sum_i128(sum, red);
     if (t == 0) bd.parts[q.part_off + d.part] = tt;
   }
+}
+
+int easy_blocks_per_cu()
+{
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_easy, 256, 0) != hipSuccess) nb = 0;
+  return nb;
 }
 
 hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s)
