@@ -59,6 +59,8 @@ struct QueryDev {
   uint64_t stage_off;                 // first entry of this query in the stage table
   uint64_t part_off;                  // first slot in the partial-sum array
   uint32_t n_ord_parts, n_triv_parts, n_easy_parts, n_hard_parts;
+  uint32_t n_p2_parts;                // P2 by table lookups: partial sums of pi(x/p) (after the hard ones)
+  uint32_t p2tab;                     // P2 from table lookups instead of P2 segment rows
   uint64_t pi_e;                      // written by setup: pi(p2_e)
   uint64_t pi_sqrtx;                  // written by setup: pi(sqrt x)
 };
@@ -159,6 +161,7 @@ struct BatchDev {
   BlockDesc *triv_desc;  uint32_t *n_triv;
   BlockDesc *easy_desc;  uint32_t *n_easy;
   BlockDesc *fin_desc;   uint32_t *n_fin;     // hard carry blocks (begin = the stage)
+  BlockDesc *p2_desc;    uint32_t *n_p2;      // P2 lookups (prime index ranges)
   HardRowInfo *rowinfo;                       // [row]
   // outputs
   uint32_t *hard_T;                           // per row, active stages: survivors before stage
@@ -185,6 +188,7 @@ hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sp
                                  uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
                                  hipStream_t s);
 int easy_blocks_per_cu();
+hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_recip(const uint32_t *primes, double *rinv, uint64_t n, hipStream_t s);
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes, uint64_t cap,
                                  hipStream_t s);

@@ -286,7 +286,7 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
-static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 16384;
+static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 16384, P2_CH = 8192;
 
 static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint64_t SB,
                        uint64_t *R, uint64_t *P, uint64_t *Y)
@@ -296,9 +296,14 @@ static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint
   *Y = std::max(*Y, h.y);
   uint64_t p = h.y;
   if ((mask & TERM_P2) && h.y < h.sqrtx) {
-    uint64_t e = h.sqrtx & ~63ull;
-    uint64_t nseg = ceil_div(h.zt + 1 - e, 2 * SB);
-    p = std::max<uint64_t>(p, cpx_isqrt128((u128)e + nseg * 2 * SB) + 1);
+    if (h.p2tab) {                  // lookups of x/p for the primes p <= sqrt x
+      *R = std::max(*R, h.zt);
+      p = std::max(p, h.sqrtx);
+    } else {                        // sieving primes of the last P2 segment
+      uint64_t e = h.sqrtx & ~63ull;
+      uint64_t nseg = ceil_div(h.zt + 1 - e, 2 * SB);
+      p = std::max<uint64_t>(p, cpx_isqrt128((u128)e + nseg * 2 * SB) + 1);
+    }
   }
   *P = std::max(*P, p);
   (void)c;
@@ -308,7 +313,7 @@ static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint
 // counters); build_plan places the shares.
 struct QPlan {
   QueryDev d{};
-  std::vector<BlockDesc> ord, triv, easy, fin;
+  std::vector<BlockDesc> ord, triv, easy, fin, p2d;
   std::vector<HardRowInfo> rowinfo;  // replicas and M rows are set per group
   std::vector<float> rleaves;        // estimated leaves of each hard row
   std::vector<float> rleafy;         // estimated leafy stages of each hard row
@@ -354,9 +359,12 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   d.pi_sqrtz = (uint32_t)host_pi(c, h.sqrtz);
   // segments of 2*SB integers from 0: the hard leaves use [0, z]; P2 counts
   // primes in (p2_e, zt] in the segments that hold them (a P2 epilogue of
-  // the same tile), so the tiles reach z, or zt when only P2 is asked for
+  // the same tile), so the tiles reach z, or zt when only P2 is asked for --
+  // unless P2 comes from pi-table lookups
+  const bool p2 = (mask & TERM_P2) && h.y < h.sqrtx;
+  d.p2tab = p2 && h.p2tab;
   d.p2_seg0 = d.p2_e / (2 * SB);
-  d.p2_nseg = ((mask & TERM_P2) && h.y < h.sqrtx) ? h.zt / (2 * SB) + 1 - d.p2_seg0 : 0;
+  d.p2_nseg = p2 && !h.p2tab ? h.zt / (2 * SB) + 1 - d.p2_seg0 : 0;
   d.hard_nseg = d.nstage ? ceil_div(h.z + 1, 2 * SB) : (d.p2_nseg ? d.p2_seg0 + d.p2_nseg : 0);
   uint32_t part = 0;
   // ordinary: odd n <= y
@@ -398,6 +406,12 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   // hard carry: one block per stage
   for (uint64_t b = 0; b < d.nstage; b++) qp.fin.push_back({i, part++, b, b + 1});
   d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
+  // P2 by lookups: prime indices (pi(y), pi(sqrt x)] (primes[k] is the (k+1)-th prime)
+  if (d.p2tab) {
+    const uint64_t k0 = d.a, k1 = host_pi(c, h.sqrtx);
+    for (uint64_t b = k0; b < k1; b += P2_CH) qp.p2d.push_back({i, part++, b, std::min(k1, b + P2_CH)});
+  }
+  d.n_p2_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts - d.n_hard_parts;
   qp.parts = part;
   // hard rows: active stages per segment and, for the low (leaf-dense)
   // segments, an estimate of the leaves to evaluate (replicas are chosen per
@@ -664,9 +678,9 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   parallel_for(pl.groups.size(), nt,
                [&](size_t g) { plan_group(c, pl.groups[g], (uint32_t)g, qps, gps[g]); });
   // place the shares: global offsets of every query's rows, parts and counters
-  struct Off { size_t ord, triv, easy, fin, rows, tiles; };
+  struct Off { size_t ord, triv, easy, fin, p2d, rows, tiles; };
   std::vector<Off> off(nq);
-  size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, nrows = 0, ntiles = 0;
+  size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, np2d = 0, nrows = 0, ntiles = 0;
   pl.q.resize(nq);
   for (size_t i = 0; i < nq; i++) {
     QPlan &qp = qps[i];
@@ -688,12 +702,12 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     pl.hard_segments += d.hard_nseg;
     pl.p2_segments += d.p2_nseg;
     pl.q[i] = d;
-    off[i] = {nord, ntriv, neasy, nfin, nrows, ntiles};
-    nord += qp.ord.size(); ntriv += qp.triv.size(); neasy += qp.easy.size();
+    off[i] = {nord, ntriv, neasy, nfin, np2d, nrows, ntiles};
+    nord += qp.ord.size(); ntriv += qp.triv.size(); neasy += qp.easy.size(); np2d += qp.p2d.size();
     nfin += qp.fin.size(); nrows += qp.rowinfo.size();
   }
   for (const GPlan &gp : gps) ntiles += gp.tiles.size();
-  pl.ord.resize(nord); pl.triv.resize(ntriv); pl.easy.resize(neasy);
+  pl.ord.resize(nord); pl.triv.resize(ntriv); pl.easy.resize(neasy); pl.p2d.resize(np2d);
   pl.fin.resize(nfin); pl.rowinfo.resize(nrows);
   // longest first: counting sort on the cost buckets (descending, stable in
   // group order within a bucket), O(tiles)
@@ -709,6 +723,7 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     std::copy(qp.ord.begin(), qp.ord.end(), pl.ord.begin() + off[i].ord);
     std::copy(qp.triv.begin(), qp.triv.end(), pl.triv.begin() + off[i].triv);
     std::copy(qp.easy.begin(), qp.easy.end(), pl.easy.begin() + off[i].easy);
+    std::copy(qp.p2d.begin(), qp.p2d.end(), pl.p2d.begin() + off[i].p2d);
     std::copy(qp.fin.begin(), qp.fin.end(), pl.fin.begin() + off[i].fin);
     std::copy(qp.rowinfo.begin(), qp.rowinfo.end(), pl.rowinfo.begin() + off[i].rows);
   });
@@ -719,11 +734,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
 // Fixed-capacity layout: array offsets depend only on capacities, so the
 // pointers baked into a captured graph stay valid until a capacity grows.
 struct Caps {
-  uint64_t q, stages, tiles, groups, ord, triv, easy, fin, tm, rows, p2rows, parts, mm, mrows;
+  uint64_t q, stages, tiles, groups, ord, triv, easy, fin, p2d, tm, rows, p2rows, parts, mm, mrows;
 };
 
 struct Layout {
-  size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, rowinfo, plan_bytes;
+  size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, p2d, rowinfo, plan_bytes;
   size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
 };
 
@@ -742,6 +757,7 @@ static Layout make_layout(const Caps &k)
   L.triv = o;     o = al(o + k.triv * sizeof(BlockDesc));
   L.easy = o;     o = al(o + k.easy * sizeof(BlockDesc));
   L.fin = o;      o = al(o + k.fin * sizeof(BlockDesc));
+  L.p2d = o;      o = al(o + k.p2d * sizeof(BlockDesc));
   L.rowinfo = o;  o = al(o + k.rows * sizeof(HardRowInfo));
   L.plan_bytes = o;
   o = 0;
@@ -764,6 +780,7 @@ static Caps needed_caps(const Plan &pl)
   Caps k;
   k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size(); k.groups = pl.groups.size();
   k.ord = pl.ord.size(); k.triv = pl.triv.size(); k.easy = pl.easy.size(); k.fin = pl.fin.size();
+  k.p2d = pl.p2d.size();
   k.tm = pl.hard_tm; k.rows = pl.hard_rows; k.p2rows = pl.p2_rows; k.parts = pl.parts;
   k.mm = pl.hard_mm; k.mrows = pl.hard_mrows;
   return k;
@@ -799,6 +816,7 @@ static int ensure_arena(cpx_ctx *c, const Plan &pl)
   fit(k.q, std::max<uint64_t>(need.q, 64)); fit(k.stages, need.stages); fit(k.tiles, need.tiles);
   fit(k.groups, std::max<uint64_t>(need.groups, 16));
   fit(k.ord, need.ord); fit(k.triv, need.triv); fit(k.easy, need.easy); fit(k.fin, need.fin);
+  fit(k.p2d, need.p2d);
   fit(k.tm, need.tm); fit(k.rows, need.rows); fit(k.p2rows, need.p2rows); fit(k.parts, need.parts);
   fit(k.mm, need.mm); fit(k.mrows, need.mrows);
   if (!grow) return CPX_OK;
@@ -835,7 +853,7 @@ static BatchDev make_batchdev(cpx_ctx *c)
   b.wheel = c->wheel.p; b.winfo = c->winfo.p;
   uint32_t *cnt = (uint32_t *)(P + L.counters);
   b.nqueries = cnt + 0; b.ntiles = cnt + 1; b.tile_head = cnt + 2;
-  b.n_ord = cnt + 3; b.n_triv = cnt + 4; b.n_easy = cnt + 5; b.n_fin = cnt + 6;
+  b.n_ord = cnt + 3; b.n_triv = cnt + 4; b.n_easy = cnt + 5; b.n_fin = cnt + 6; b.n_p2 = cnt + 7;
   b.queries = (QueryDev *)(P + L.queries);
   b.stages = (StageDev *)(P + L.stages);
   b.tiles = (TileDev *)(P + L.tiles);
@@ -843,6 +861,7 @@ static BatchDev make_batchdev(cpx_ctx *c)
   b.ord_desc = (BlockDesc *)(P + L.ord);
   b.triv_desc = (BlockDesc *)(P + L.triv);
   b.easy_desc = (BlockDesc *)(P + L.easy);
+  b.p2_desc = (BlockDesc *)(P + L.p2d);
   b.fin_desc = (BlockDesc *)(P + L.fin);
   b.rowinfo = (HardRowInfo *)(P + L.rowinfo);
   b.hard_T = (uint32_t *)(O + L.hard_T);
@@ -867,6 +886,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   cnt[0] = (uint32_t)pl.q.size(); cnt[1] = (uint32_t)pl.tiles.size(); cnt[2] = 0;
   cnt[3] = (uint32_t)pl.ord.size(); cnt[4] = (uint32_t)pl.triv.size();
   cnt[5] = (uint32_t)pl.easy.size(); cnt[6] = (uint32_t)pl.fin.size();
+  cnt[7] = (uint32_t)pl.p2d.size();
   auto put = [&](size_t off, const void *src, size_t bytes) {
     if (bytes) memcpy(H + off, src, bytes);
   };
@@ -876,6 +896,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   put(L.ord, pl.ord.data(), pl.ord.size() * sizeof(BlockDesc));
   put(L.triv, pl.triv.data(), pl.triv.size() * sizeof(BlockDesc));
   put(L.easy, pl.easy.data(), pl.easy.size() * sizeof(BlockDesc));
+  put(L.p2d, pl.p2d.data(), pl.p2d.size() * sizeof(BlockDesc));
   put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
   put(L.rowinfo, pl.rowinfo.data(), pl.rowinfo.size() * sizeof(HardRowInfo));
   // copy only the used prefix of each array
@@ -890,6 +911,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   HIPCHK(up(L.ord, pl.ord.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.triv, pl.triv.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.easy, pl.easy.size() * sizeof(BlockDesc)));
+  HIPCHK(up(L.p2d, pl.p2d.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.fin, pl.fin.size() * sizeof(BlockDesc)));
   HIPCHK(up(L.rowinfo, pl.rowinfo.size() * sizeof(HardRowInfo)));
   return CPX_OK;
@@ -934,6 +956,7 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
     HIPCHK(launch_ordinary(b, GRID_T, c->stream));
     HIPCHK(launch_trivial(b, GRID_T, c->stream));
     HIPCHK(launch_easy(b, easy_grid(c), c->stream));
+    HIPCHK(launch_p2tab(b, GRID_T, c->stream));
   } else if (part == 1) {
     HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
   } else {
@@ -996,11 +1019,35 @@ uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &h)
   return (uint64_t)bytes;
 }
 
-int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask,
+// P2 by lookups in a resident pi table to x/(y+1) (instead of sieving the
+// P2 segments) for x < 2^64 with x/(y+1) < P2TAB_LIMIT: chosen for a batch
+// of at least P2TAB_MIN such queries, whose table build (about the cost of
+// sieving one query's P2 range) is shared, or when the resident table
+// already covers the queries.
+static const uint64_t P2TAB_LIMIT = 1ull << 35;
+static const size_t P2TAB_MIN = 8;
+
+static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t mask)
+{
+  size_t n = 0;
+  auto eligible = [&](const HostQuery &h) {
+    return !h.small && (mask & TERM_P2) && h.y < h.sqrtx && h.sqrtx < (1ull << 32) && h.zt < P2TAB_LIMIT;
+  };
+  for (const HostQuery &h : hq)
+    if (eligible(h)) n++;
+  const bool batch = n >= P2TAB_MIN;
+  for (HostQuery &h : hq)
+    h.p2tab = eligible(h) && (batch || (!c->opts.fresh_tables && c->pitab_limit >= h.zt &&
+                                        c->prime_limit >= h.sqrtx));
+}
+
+int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask,
                 std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph)
 {
   double t0 = now_ms();
-  if (hq.empty()) { res.clear(); return CPX_OK; }
+  if (hq_in.empty()) { res.clear(); return CPX_OK; }
+  std::vector<HostQuery> hq(hq_in);
+  choose_p2tab(c, hq, mask);
   // tables first (planning needs the host prime list)
   uint64_t R = 0, P = 0, Y = 0;
   const uint64_t SB = c->shape.seg_bits();

@@ -22,6 +22,7 @@ struct HostQuery {
   uint64_t y, z, zt, sqrtx, cbrtx, qrtx, sqrtz;
   uint32_t c;
   bool small;               // answered from the pi table (x < CPX_SMALL_X)
+  bool p2tab;               // P2 by pi-table lookups of x/p (the table covers x/(y+1))
 };
 
 // Which terms a plan evaluates.
@@ -36,7 +37,7 @@ struct Plan {
   std::vector<QueryDev> q;
   std::vector<TileDev> tiles;
   std::vector<GroupDev> groups;
-  std::vector<BlockDesc> ord, triv, easy, fin;
+  std::vector<BlockDesc> ord, triv, easy, fin, p2d;
   std::vector<HardRowInfo> rowinfo;
   uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
   uint64_t hard_mrows = 0, hard_mm = 0;

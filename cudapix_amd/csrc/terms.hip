@@ -217,6 +217,38 @@ hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s)
   return hipGetLastError();
 }
 
+// ------------------------------------------------------- P2 by lookups
+// Sum of pi(x/p) over a range of P2 primes p = primes[k] (y < p <= sqrt x,
+// x < 2^64), one block per descriptor: x/p by the reciprocal divide with the
+// tabulated 1/p (x/p <= x/(y+1) < 2^50), pi from the resident table, which
+// reaches x/(y+1).  The same sum the reference takes with upper_bound over
+// its window prime lists (src/P2.cu:76-133); k_combine subtracts
+// sum (pi(p) - 1) in closed form.
+__global__ __launch_bounds__(256) void k_p2tab(BatchDev bd)
+{
+  __shared__ i128 red[4];
+  const uint32_t nd = *bd.n_p2;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const BlockDesc d = bd.p2_desc[di];
+    const QueryDev &q = bd.queries[d.query];
+    const uint64_t x = q.x_lo;
+    const double xf = (double)x;
+    uint64_t s = 0;
+    for (uint64_t k = d.begin + threadIdx.x; k < d.end; k += blockDim.x) {
+      const uint64_t v = cpx_div_leaf_r(x, xf, bd.primes[k], bd.prinv[k]);
+      s += pi_lookup(bd.pitab, v);
+    }
+    const i128 tt = block_sum_i128((i128)s, red);
+    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = tt;
+  }
+}
+
+hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_p2tab, dim3(grid), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
 // -------------------------------------------------------- hard carry
 // Hard-leaf carry of one stage s of one query per block:
 //   carry_s = sum over segments r (in order, while s is active in r) of
@@ -335,20 +367,23 @@ __global__ __launch_bounds__(256) void k_combine(BatchDev bd)
       res.pi_y = q.a;
       res.status = 1;
     }
-    // the term kernels' block partial sums, by term (ordinary | trivial U | easy | hard carry)
-    i128 ps0 = 0, psu = 0, psv = 0, psh = 0;
+    // the term kernels' block partial sums, by term (ordinary | trivial U |
+    // easy | hard carry | P2 lookups)
+    i128 ps0 = 0, psu = 0, psv = 0, psh = 0, psp = 0;
     if (!q.small) {
       const i128 *pp = bd.parts + q.part_off;
       const uint32_t n0 = q.n_ord_parts, n1 = n0 + q.n_triv_parts, n2 = n1 + q.n_easy_parts,
-                     n3 = n2 + q.n_hard_parts;
-      for (uint32_t i = t; i < n3; i += blockDim.x) {
+                     n3 = n2 + q.n_hard_parts, n4 = n3 + q.n_p2_parts;
+      for (uint32_t i = t; i < n4; i += blockDim.x) {
         const i128 v = pp[i];
         if (i < n0) ps0 += v;
         else if (i < n1) psu += v;
         else if (i < n2) psv += v;
-        else psh += v;
+        else if (i < n3) psh += v;
+        else psp += v;
       }
     }
+    psp = block_sum_i128(psp, red);
     ps0 = block_sum_i128(ps0, red);
     psu = block_sum_i128(psu, red);
     psv = block_sum_i128(psv, red);
@@ -363,7 +398,7 @@ __global__ __launch_bounds__(256) void k_combine(BatchDev bd)
       // P2 = sum pi(x/p) - sum_{p} (pi(p) - 1), pi(p) = a+1 .. A
       i128 A = (i128)q.pi_sqrtx, a = (i128)q.a;
       i128 P2 = 0;
-      if (A > a) P2 = p2s - (A * (A - 1) / 2 - a * (a - 1) / 2);
+      if (A > a) P2 = p2s + psp - (A * (A - 1) / 2 - a * (a - 1) / 2);
       i128 pi = s0 + triv + v + hard + a - 1 - P2;
       ResultDev &res = bd.results[qi];
       put_i128(res.pi, pi);

@@ -49,6 +49,23 @@ def test_batch_terms_match_oracle(ctx, oracle):
             assert t[k] == ref[k], (x, k)
 
 
+def test_p2_lookup_equals_p2_segments():
+    """P2 from lookups in a resident pi table (a batch of >= 8 queries) equals
+    P2 counted in the sieve's P2 segments (one query in a context with fresh
+    tables), term by term."""
+    import cudapix_amd as C
+    xs = [x for x in c3() if 10 ** 11 < x < 10 ** 14][:12]
+    assert len(xs) >= 8
+    with C.Context(device=0) as a:
+        batch = a.pi_batch(xs, terms=True)
+    with C.Context(device=0, fresh_tables=True) as b:
+        for x, t in zip(xs, batch):
+            alone = b.pi_batch([x], terms=True)[0]
+            assert alone["y"] == t["y"]
+            for k in ("p2", "pi"):
+                assert alone[k] == t[k], (x, k)
+
+
 @pytest.mark.parametrize("shape", [(64, 32), (64, 64), (64, 128), (128, 32), (128, 64), (256, 32),
                                    (256, 64), (512, 32)])
 def test_segment_shapes_agree(oracle, shape):
