@@ -45,7 +45,7 @@ class CpxTerms(ctypes.Structure):
 
 
 class CpxBatchOpts(ctypes.Structure):
-    _fields_ = [("use_graph", ctypes.c_uint32), ("tune", ctypes.c_uint32)]
+    _fields_ = [("use_graph", ctypes.c_uint32), ("tune", ctypes.c_uint32), ("p2_table", ctypes.c_uint32)]
 
 
 class CpxBatchStats(ctypes.Structure):
@@ -360,11 +360,14 @@ class Context:
             hip.hipFree(d)
 
     # ---- batch
-    def pi_batch(self, xs, use_graph=True, tune=False, terms=False, stats=False):
+    def pi_batch(self, xs, use_graph=True, tune=False, terms=False, stats=False, p2_table=0):
+        """pi(x) for every x in one launch sequence.  p2_table: 0 = P2 by lookups
+        in the resident pi table when >= 8 queries qualify (the table grows),
+        1 = only where the resident table reaches, 2 = never."""
         n = len(xs)
         arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
         out = (CpxU128 * max(n, 1))()
-        bo = CpxBatchOpts(int(use_graph), int(tune))
+        bo = CpxBatchOpts(int(use_graph), int(tune), int(p2_table))
         st = CpxBatchStats()
         if terms:
             tt = (CpxTerms * max(n, 1))()

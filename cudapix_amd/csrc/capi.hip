@@ -144,7 +144,7 @@ static int solve_one(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, uint32_t mask,
   if (rc) return rc;
   std::vector<HostQuery> v{hq};
   std::vector<ResultDev> res;
-  if ((rc = run_queries(c, v, mask, res, nullptr, false))) return rc;
+  if ((rc = run_queries(c, v, mask, 0, res, nullptr, false))) return rc;
   *r = res[0];
   if (hq_out) *hq_out = hq;
   if (ms) *ms = now_ms() - t0;
@@ -621,7 +621,7 @@ static int batch_impl(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out, c
   std::vector<ResultDev> res;
   bool graph = bo ? bo->use_graph != 0 : true;
   cpx_batch_stats local{};
-  int rc = run_queries(c, hq, TERM_ALL, res, st ? &local : nullptr, graph);
+  int rc = run_queries(c, hq, TERM_ALL, bo ? bo->p2_table : 0, res, st ? &local : nullptr, graph);
   if (rc) return rc;
   for (size_t i = 0; i < n; i++) {
     out[i] = {res[i].pi[0], res[i].pi[1]};
@@ -671,10 +671,10 @@ int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, ui
   // time one configuration (after a warm-up run for tables and arena)
   auto trial = [&](double &ms) -> int {
     int rc = make();
-    if (!rc) rc = run_queries(c, hq, TERM_ALL, res, nullptr, false);
+    if (!rc) rc = run_queries(c, hq, TERM_ALL, 0, res, nullptr, false);
     if (rc) return rc;
     const double t0 = now_ms();
-    if ((rc = run_queries(c, hq, TERM_ALL, res, nullptr, false))) return rc;
+    if ((rc = run_queries(c, hq, TERM_ALL, 0, res, nullptr, false))) return rc;
     ms = now_ms() - t0;
     if (!have_ref) { ref = res; have_ref = true; }
     for (size_t i = 0; i < n; i++)

@@ -1027,7 +1027,7 @@ uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &h)
 static const uint64_t P2TAB_LIMIT = 1ull << 35;
 static const size_t P2TAB_MIN = 8;
 
-static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t mask)
+static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode)
 {
   size_t n = 0;
   auto eligible = [&](const HostQuery &h) {
@@ -1035,19 +1035,19 @@ static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t 
   };
   for (const HostQuery &h : hq)
     if (eligible(h)) n++;
-  const bool batch = n >= P2TAB_MIN;
+  const bool batch = p2mode == 0 && n >= P2TAB_MIN;
   for (HostQuery &h : hq)
-    h.p2tab = eligible(h) && (batch || (!c->opts.fresh_tables && c->pitab_limit >= h.zt &&
-                                        c->prime_limit >= h.sqrtx));
+    h.p2tab = p2mode != 2 && eligible(h) &&
+              (batch || (!c->opts.fresh_tables && c->pitab_limit >= h.zt && c->prime_limit >= h.sqrtx));
 }
 
-int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask,
+int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask, uint32_t p2mode,
                 std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph)
 {
   double t0 = now_ms();
   if (hq_in.empty()) { res.clear(); return CPX_OK; }
   std::vector<HostQuery> hq(hq_in);
-  choose_p2tab(c, hq, mask);
+  choose_p2tab(c, hq, mask, p2mode);
   // tables first (planning needs the host prime list)
   uint64_t R = 0, P = 0, Y = 0;
   const uint64_t SB = c->shape.seg_bits();

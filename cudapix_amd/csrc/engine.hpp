@@ -101,7 +101,8 @@ int hip_fail(cpx_ctx *c, hipError_t e, const char *what);
 uint64_t choose_y_host(const cpx_ctx *c, u128 x);
 int make_host_query(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, HostQuery *hq);
 int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y);
-int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask,
+// p2mode: cpx_batch_opts.p2_table
+int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
                 std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph);
 uint64_t host_pi(const cpx_ctx *c, uint64_t v);   // v <= prime_limit
 uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &hq);

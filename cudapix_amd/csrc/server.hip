@@ -120,7 +120,9 @@ static void lane_loop(cpx_server *s, cpx_ctx *ctx)
     xs.resize(grp.size());
     out.resize(grp.size());
     for (size_t i = 0; i < grp.size(); i++) xs[i] = grp[i].x;
-    cpx_batch_opts bo{1, 0};
+    // the P2 table only where it already reaches: a growing table would
+    // stall the stream (and is not in the admission budget)
+    cpx_batch_opts bo{1, 0, 1};
     int rc = cpx_pi_batch(ctx, xs.data(), xs.size(), out.data(), &bo, nullptr);
     double t_done = srv_now_ms();
     {

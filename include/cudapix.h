@@ -152,6 +152,10 @@ int cpx_free(cpx_ctx *ctx, void *d_ptr);
 typedef struct {
   uint32_t use_graph;      /* 1: replay a captured hipGraph of the batch launches */
   uint32_t tune;           /* 1: let the autotuner pick the segment shape first */
+  uint32_t p2_table;       /* P2 of x < 2^64 by lookups in the resident pi table, extended to
+                            * x/(y+1): 0 when at least 8 queries qualify (the table grows),
+                            * 1 only where the resident table already reaches, 2 never
+                            * (P2 is then counted in the sieve's P2 segments) */
 } cpx_batch_opts;
 
 typedef struct {

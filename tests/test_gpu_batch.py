@@ -50,20 +50,26 @@ def test_batch_terms_match_oracle(ctx, oracle):
 
 
 def test_p2_lookup_equals_p2_segments():
-    """P2 from lookups in a resident pi table (a batch of >= 8 queries) equals
-    P2 counted in the sieve's P2 segments (one query in a context with fresh
-    tables), term by term."""
+    """P2 from lookups in the resident pi table (a batch of >= 8 queries)
+    equals P2 counted in the sieve's P2 segments (p2_table=2), term by term,
+    also in one batch with both kinds of query (p2_table=1 after the table
+    reached half of them)."""
     import cudapix_amd as C
     xs = [x for x in c3() if 10 ** 11 < x < 10 ** 14][:12]
     assert len(xs) >= 8
     with C.Context(device=0) as a:
-        batch = a.pi_batch(xs, terms=True)
-    with C.Context(device=0, fresh_tables=True) as b:
-        for x, t in zip(xs, batch):
-            alone = b.pi_batch([x], terms=True)[0]
-            assert alone["y"] == t["y"]
+        looked = a.pi_batch(xs, terms=True, p2_table=0)
+        sieved = a.pi_batch(xs, terms=True, p2_table=2)
+        for x, t, u in zip(xs, looked, sieved):
+            assert t["y"] == u["y"]
             for k in ("p2", "pi"):
-                assert alone[k] == t[k], (x, k)
+                assert t[k] == u[k], (x, k)
+    with C.Context(device=0) as b:
+        small = sorted(xs)[:8]
+        b.pi_batch(small, p2_table=0)             # the table reaches the smaller half
+        mixed = b.pi_batch(xs, terms=True, p2_table=1)
+        assert [t["pi"] for t in mixed] == [t["pi"] for t in looked]
+        assert [t["p2"] for t in mixed] == [t["p2"] for t in looked]
 
 
 @pytest.mark.parametrize("shape", [(64, 32), (64, 64), (64, 128), (128, 32), (128, 64), (256, 32),
