@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--serve-n", type=int, default=400)
     ap.add_argument("--serve-lambdas", default="200,2000")
     ap.add_argument("--serve-xmax", type=float, default=1e16)
+    ap.add_argument("--serve-full", action="store_true",
+                    help="every stream of the served fixture: lambda 50..2000, 2000 requests, x < 1e17")
     ap.add_argument("--profile", choices=["single", "batch", "copy"],
                     help="run only one workload --reps times (for rocprofv3) and exit")
     ap.add_argument("--x", type=float, default=1e16, help="x of --profile single")
@@ -237,6 +239,8 @@ def main():
     ap.add_argument("--tuning", default="",
                     help="threads,kwords,alpha_scale,c for --profile batch (as the timed run's autotune chose)")
     args = ap.parse_args()
+    if args.serve_full:
+        args.serve_lambdas, args.serve_n, args.serve_xmax = "50,200,500,1000,2000", 2000, 1e17
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and this "
               "workload shards by query with no exchange (see DESIGN.md); run with --gpus 1",
@@ -362,9 +366,16 @@ def main():
             srv.close()
             got = {r["ticket"]: r for r in res}
             ok = all(r["status"] == 0 for r in res) and len(res) == len(sx)
+            waits = sorted(r["t_admit_ms"] - r["t_arrive_ms"] for r in res)
+            qw = (lambda f: round(waits[min(len(waits) - 1, int(f * len(waits)))], 3)) if waits else \
+                (lambda f: None)
             served[str(lam)] = {"n": len(sx), "qps": round(stt["qps"], 2),
                                 "p50_ms": round(stt["p50_ms"], 3), "p99_ms": round(stt["p99_ms"], 3),
-                                "batches": stt["batches"], "all_ok": ok}
+                                "mean_ms": round(stt["mean_ms"], 3),
+                                "queue_p50_ms": qw(0.5), "queue_p99_ms": qw(0.99),
+                                "submitted": stt["submitted"], "completed": stt["completed"],
+                                "rejected": stt["rejected"], "batches": stt["batches"],
+                                "max_inflight_bytes": stt["max_inflight_bytes"], "all_ok": ok}
             log("served lambda=%d: %s" % (lam, served[str(lam)]))
         # each served result equals the same x solved alone
         out["served"] = served
