@@ -50,3 +50,19 @@ def test_pi_1e20_two_alphas(ctx):
         print("case pi(1e20) alpha=%g y=%d" % (alpha, y), flush=True)
         t = ctx.pi(x, y=y, terms=True)
         assert t["pi"] == KNOWN["pi10"]["20"], (alpha, t)
+
+
+def test_batch_known_values(ctx):
+    # one batch (query groups, P2 by table lookups) of published values
+    xs = [10 ** n for n in range(9, 16)] + [2 ** k for k in (30, 32, 40)]
+    want = [KNOWN["pi10"][str(n)] for n in range(9, 16)] + [KNOWN["pi2"][str(k)] for k in (30, 32, 40)]
+    assert ctx.pi_batch(xs) == want
+
+
+@pytest.mark.slow
+def test_batch_with_u128_member(ctx):
+    # a 128-bit x (P2 from its sieve segments, a group of its own) in a batch
+    # whose other members take P2 from table lookups
+    xs = [2 ** 64] + [10 ** n for n in range(9, 17)]
+    want = [KNOWN["pi2"]["64"]] + [KNOWN["pi10"][str(n)] for n in range(9, 17)]
+    assert ctx.pi_batch(xs) == want
