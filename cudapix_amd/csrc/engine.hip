@@ -326,11 +326,39 @@ struct GPlan {
   std::vector<uint16_t> tbucket;     // cost bucket of each tile (longest first)
 };
 
+// Where one query's descriptors and rows go in the batch arrays.
+struct PlanOff { size_t ord, triv, easy, fin, p2d, rows; };
+
+// A batch plan: queries, groups and sorted tiles; the per-query descriptors
+// and rows stay in the query shares and are written straight into the
+// pinned upload buffer at their offsets.
+struct Plan {
+  std::vector<QueryDev> q;
+  std::vector<TileDev> tiles;
+  std::vector<GroupDev> groups;
+  std::vector<QPlan> qps;
+  std::vector<PlanOff> off;
+  size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, np2d = 0, nrows = 0;
+  uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
+  uint64_t hard_mrows = 0, hard_mm = 0;
+  uint64_t hard_segments = 0, p2_segments = 0;
+};
+
 // Cost buckets: 1/8 octave of the tile's cost estimate.
 static const int PLAN_NB = 512;
 static inline uint16_t cost_bucket(double cost)
 {
-  int b = (int)std::floor(std::log2(std::max(cost, 1.0)) * 8.0);
+  // floor(8 log2 cost) from the exponent and the top mantissa bits (the
+  // eighth-octave thresholds of [1, 2) are 2^(k/8))
+  static const double thr[8] = {1.0, 1.0905077326652577, 1.1892071150027210, 1.2968395546510096,
+                                1.4142135623730951, 1.5422108254079407, 1.6817928305074290,
+                                1.8340080864093424};
+  if (!(cost > 1.0)) return 0;
+  int e;
+  const double m = 2.0 * std::frexp(cost, &e);       // cost = m * 2^(e-1), m in [1, 2)
+  int k = 7;
+  while (k > 0 && m < thr[k]) k--;
+  const int b = 8 * (e - 1) + k;
   return (uint16_t)std::min(std::max(b, 0), PLAN_NB - 1);
 }
 
@@ -669,7 +697,8 @@ static unsigned plan_threads(size_t nq)
 static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl)
 {
   const size_t nq = hq.size();
-  std::vector<QPlan> qps(nq);
+  std::vector<QPlan> &qps = pl.qps;
+  qps.assign(nq, QPlan());
   const unsigned nt = plan_threads(nq);
   parallel_for(nq, nt, [&](size_t i) { plan_query(c, hq[i], (uint32_t)i, mask, qps[i]); });
   pl.groups.clear();
@@ -678,9 +707,9 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   parallel_for(pl.groups.size(), nt,
                [&](size_t g) { plan_group(c, pl.groups[g], (uint32_t)g, qps, gps[g]); });
   // place the shares: global offsets of every query's rows, parts and counters
-  struct Off { size_t ord, triv, easy, fin, p2d, rows, tiles; };
-  std::vector<Off> off(nq);
-  size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, np2d = 0, nrows = 0, ntiles = 0;
+  std::vector<PlanOff> &off = pl.off;
+  off.resize(nq);
+  size_t ntiles = 0;
   pl.q.resize(nq);
   for (size_t i = 0; i < nq; i++) {
     QPlan &qp = qps[i];
@@ -702,13 +731,11 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
     pl.hard_segments += d.hard_nseg;
     pl.p2_segments += d.p2_nseg;
     pl.q[i] = d;
-    off[i] = {nord, ntriv, neasy, nfin, np2d, nrows, ntiles};
-    nord += qp.ord.size(); ntriv += qp.triv.size(); neasy += qp.easy.size(); np2d += qp.p2d.size();
-    nfin += qp.fin.size(); nrows += qp.rowinfo.size();
+    off[i] = {pl.nord, pl.ntriv, pl.neasy, pl.nfin, pl.np2d, pl.nrows};
+    pl.nord += qp.ord.size(); pl.ntriv += qp.triv.size(); pl.neasy += qp.easy.size();
+    pl.np2d += qp.p2d.size(); pl.nfin += qp.fin.size(); pl.nrows += qp.rowinfo.size();
   }
   for (const GPlan &gp : gps) ntiles += gp.tiles.size();
-  pl.ord.resize(nord); pl.triv.resize(ntriv); pl.easy.resize(neasy); pl.p2d.resize(np2d);
-  pl.fin.resize(nfin); pl.rowinfo.resize(nrows);
   // longest first: counting sort on the cost buckets (descending, stable in
   // group order within a bucket), O(tiles)
   std::vector<uint32_t> start(PLAN_NB + 1, 0);
@@ -718,15 +745,6 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   pl.tiles.resize(ntiles);
   for (const GPlan &gp : gps)
     for (size_t k = 0; k < gp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - gp.tbucket[k]]++] = gp.tiles[k];
-  parallel_for(nq, nt, [&](size_t i) {
-    const QPlan &qp = qps[i];
-    std::copy(qp.ord.begin(), qp.ord.end(), pl.ord.begin() + off[i].ord);
-    std::copy(qp.triv.begin(), qp.triv.end(), pl.triv.begin() + off[i].triv);
-    std::copy(qp.easy.begin(), qp.easy.end(), pl.easy.begin() + off[i].easy);
-    std::copy(qp.p2d.begin(), qp.p2d.end(), pl.p2d.begin() + off[i].p2d);
-    std::copy(qp.fin.begin(), qp.fin.end(), pl.fin.begin() + off[i].fin);
-    std::copy(qp.rowinfo.begin(), qp.rowinfo.end(), pl.rowinfo.begin() + off[i].rows);
-  });
   return CPX_OK;
 }
 
@@ -779,8 +797,8 @@ static Caps needed_caps(const Plan &pl)
 {
   Caps k;
   k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size(); k.groups = pl.groups.size();
-  k.ord = pl.ord.size(); k.triv = pl.triv.size(); k.easy = pl.easy.size(); k.fin = pl.fin.size();
-  k.p2d = pl.p2d.size();
+  k.ord = pl.nord; k.triv = pl.ntriv; k.easy = pl.neasy; k.fin = pl.nfin;
+  k.p2d = pl.np2d;
   k.tm = pl.hard_tm; k.rows = pl.hard_rows; k.p2rows = pl.p2_rows; k.parts = pl.parts;
   k.mm = pl.hard_mm; k.mrows = pl.hard_mrows;
   return k;
@@ -884,21 +902,26 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   uint32_t *cnt = (uint32_t *)(H + L.counters);
   memset(cnt, 0, 64);
   cnt[0] = (uint32_t)pl.q.size(); cnt[1] = (uint32_t)pl.tiles.size(); cnt[2] = 0;
-  cnt[3] = (uint32_t)pl.ord.size(); cnt[4] = (uint32_t)pl.triv.size();
-  cnt[5] = (uint32_t)pl.easy.size(); cnt[6] = (uint32_t)pl.fin.size();
-  cnt[7] = (uint32_t)pl.p2d.size();
+  cnt[3] = (uint32_t)pl.nord; cnt[4] = (uint32_t)pl.ntriv;
+  cnt[5] = (uint32_t)pl.neasy; cnt[6] = (uint32_t)pl.nfin;
+  cnt[7] = (uint32_t)pl.np2d;
   auto put = [&](size_t off, const void *src, size_t bytes) {
     if (bytes) memcpy(H + off, src, bytes);
   };
   put(L.queries, pl.q.data(), pl.q.size() * sizeof(QueryDev));
   put(L.tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDev));
   put(L.groups, pl.groups.data(), pl.groups.size() * sizeof(GroupDev));
-  put(L.ord, pl.ord.data(), pl.ord.size() * sizeof(BlockDesc));
-  put(L.triv, pl.triv.data(), pl.triv.size() * sizeof(BlockDesc));
-  put(L.easy, pl.easy.data(), pl.easy.size() * sizeof(BlockDesc));
-  put(L.p2d, pl.p2d.data(), pl.p2d.size() * sizeof(BlockDesc));
-  put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
-  put(L.rowinfo, pl.rowinfo.data(), pl.rowinfo.size() * sizeof(HardRowInfo));
+  // every query's descriptors and rows straight to their places
+  parallel_for(pl.qps.size(), plan_threads(pl.qps.size()), [&](size_t i) {
+    const QPlan &qp = pl.qps[i];
+    const PlanOff &o = pl.off[i];
+    put(L.ord + o.ord * sizeof(BlockDesc), qp.ord.data(), qp.ord.size() * sizeof(BlockDesc));
+    put(L.triv + o.triv * sizeof(BlockDesc), qp.triv.data(), qp.triv.size() * sizeof(BlockDesc));
+    put(L.easy + o.easy * sizeof(BlockDesc), qp.easy.data(), qp.easy.size() * sizeof(BlockDesc));
+    put(L.p2d + o.p2d * sizeof(BlockDesc), qp.p2d.data(), qp.p2d.size() * sizeof(BlockDesc));
+    put(L.fin + o.fin * sizeof(BlockDesc), qp.fin.data(), qp.fin.size() * sizeof(BlockDesc));
+    put(L.rowinfo + o.rows * sizeof(HardRowInfo), qp.rowinfo.data(), qp.rowinfo.size() * sizeof(HardRowInfo));
+  });
   // copy only the used prefix of each array
   auto up = [&](size_t off, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
@@ -908,12 +931,12 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   HIPCHK(up(L.queries, pl.q.size() * sizeof(QueryDev)));
   HIPCHK(up(L.tiles, pl.tiles.size() * sizeof(TileDev)));
   HIPCHK(up(L.groups, pl.groups.size() * sizeof(GroupDev)));
-  HIPCHK(up(L.ord, pl.ord.size() * sizeof(BlockDesc)));
-  HIPCHK(up(L.triv, pl.triv.size() * sizeof(BlockDesc)));
-  HIPCHK(up(L.easy, pl.easy.size() * sizeof(BlockDesc)));
-  HIPCHK(up(L.p2d, pl.p2d.size() * sizeof(BlockDesc)));
-  HIPCHK(up(L.fin, pl.fin.size() * sizeof(BlockDesc)));
-  HIPCHK(up(L.rowinfo, pl.rowinfo.size() * sizeof(HardRowInfo)));
+  HIPCHK(up(L.ord, pl.nord * sizeof(BlockDesc)));
+  HIPCHK(up(L.triv, pl.ntriv * sizeof(BlockDesc)));
+  HIPCHK(up(L.easy, pl.neasy * sizeof(BlockDesc)));
+  HIPCHK(up(L.p2d, pl.np2d * sizeof(BlockDesc)));
+  HIPCHK(up(L.fin, pl.nfin * sizeof(BlockDesc)));
+  HIPCHK(up(L.rowinfo, pl.nrows * sizeof(HardRowInfo)));
   return CPX_OK;
 }
 

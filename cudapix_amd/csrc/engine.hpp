@@ -33,17 +33,6 @@ enum : uint32_t {
   TERM_EASY_B = 64    // restrict V to sqrt(x/y) < p <= cbrt(x)   (V_b, src/V.cu:89-124)
 };
 
-struct Plan {
-  std::vector<QueryDev> q;
-  std::vector<TileDev> tiles;
-  std::vector<GroupDev> groups;
-  std::vector<BlockDesc> ord, triv, easy, fin, p2d;
-  std::vector<HardRowInfo> rowinfo;
-  uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
-  uint64_t hard_mrows = 0, hard_mm = 0;
-  uint64_t need_R = 0, need_P = 0, need_Y = 0;
-  uint64_t hard_segments = 0, p2_segments = 0;
-};
 
 struct cpx_ctx {
   int device = 0;
