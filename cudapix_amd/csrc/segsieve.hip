@@ -765,13 +765,13 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
         uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
         const uint32_t a0 = t * (K + 1);
         uint32_t wd;
+        // (one LDS and-with-return per multiple: the old word says whether
+        // the bit was still set; the words are the thread's own)
         if (p > BPT) {                                    // at most one multiple per thread
           uint32_t hit = 0;
           if (j < BPT) {
-            const uint32_t addr = a0 + (j >> 5);
-            const uint32_t bit = 1u << (j & 31u);
-            const uint32_t w = lds[addr];
-            if (w & bit) { lds[addr] = w & ~bit; hit = 1; }
+            const uint32_t old = atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31u)));
+            hit = (old >> (j & 31u)) & 1u;
           }
           cnt -= hit;
           cnt_lo -= hit & (j < BPT / 2 ? 1u : 0u);
@@ -779,10 +779,10 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
         } else {
           uint32_t d = 0, dlo = 0;
           for (; j < BPT; j += p) {
-            const uint32_t addr = a0 + (j >> 5);
-            const uint32_t bit = 1u << (j & 31u);
-            const uint32_t w = lds[addr];
-            if (w & bit) { lds[addr] = w & ~bit; d++; dlo += j < BPT / 2; }
+            const uint32_t old = atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31u)));
+            const uint32_t h = (old >> (j & 31u)) & 1u;
+            d += h;
+            dlo += j < BPT / 2 ? h : 0u;
           }
           cnt -= d;
           cnt_lo -= dlo;
