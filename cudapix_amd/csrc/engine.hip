@@ -23,6 +23,7 @@
 #include <functional>
 #include <mutex>
 #include <thread>
+#include <unistd.h>
 
 static double now_ms()
 {
@@ -595,12 +596,14 @@ static void plan_group(const cpx_ctx *c, const GroupDev &g, uint32_t gi, std::ve
 // threads per call would dominate it).  run(n, threads, f) runs f(k) for k
 // in [0, n) on the caller and up to threads - 1 workers.  Jobs are
 // serialised by a mutex: contexts on several host threads plan one at a time.
+// The pool is never destroyed (its idle workers end with the process), and
+// a forked child, which has no workers, runs every job inline.
 class PlanPool {
 public:
   static PlanPool &get()
   {
-    static PlanPool pool;
-    return pool;
+    static PlanPool *pool = new PlanPool();
+    return *pool;
   }
   template <typename F>
   void run(size_t n, unsigned threads, F &f)
@@ -608,6 +611,7 @@ public:
     if (n == 0) return;
     threads = (unsigned)std::min<size_t>(std::max(1u, threads), n);
     threads = std::min<unsigned>(threads, (unsigned)workers_.size() + 1);
+    if (getpid() != pid_) threads = 1;       // a forked child has no workers
     if (threads == 1) {
       for (size_t k = 0; k < n; k++) f(k);
       return;
@@ -633,21 +637,11 @@ public:
   }
 
 private:
-  PlanPool()
+  PlanPool() : pid_(getpid())
   {
     unsigned hc = std::thread::hardware_concurrency();
     unsigned nw = std::min<unsigned>(hc ? hc : 4, 16);
     for (unsigned i = 1; i < nw; i++) workers_.emplace_back([this] { loop(); });
-  }
-  ~PlanPool()
-  {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-      gen_++;
-    }
-    cv_.notify_all();
-    for (auto &w : workers_) w.join();
   }
   void loop()
   {
@@ -656,8 +650,7 @@ private:
       std::function<void()> *job;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-        if (stop_) return;
+        cv_.wait(lk, [&] { return gen_ != seen; });
         seen = gen_;
         if (taken_gen_ != gen_) { taken_gen_ = gen_; taken_ = 0; }
         if (taken_ >= want_) continue;     // enough workers joined this job
@@ -672,13 +665,13 @@ private:
       done_cv_.notify_one();
     }
   }
+  const pid_t pid_;
   std::vector<std::thread> workers_;
   std::mutex job_mu_, mu_;
   std::condition_variable cv_, done_cv_;
   std::function<void()> *job_ = nullptr;
   uint64_t gen_ = 0, taken_gen_ = 0;
   unsigned want_ = 0, taken_ = 0, done_ = 0;
-  bool stop_ = false;
 };
 
 template <typename F>
