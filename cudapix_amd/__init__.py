@@ -94,6 +94,7 @@ EXPORTS = [
     "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave", "cpx_probe_peaks",
     "cpx_get_tuning",
     "cpx_set_tuning",
+    "cpx_pi_batch_submit", "cpx_pi_batch_collect",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -163,6 +164,9 @@ def lib():
                                               P(CpxTerms), P(CpxBatchOpts), P(CpxBatchStats)]),
         "cpx_autotune": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(ctypes.c_uint32),
                                         P(ctypes.c_uint32)]),
+        "cpx_pi_batch_submit": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxBatchOpts),
+                                               P(U64)]),
+        "cpx_pi_batch_collect": (ctypes.c_int, [vp, U64, P(CpxU128), P(CpxBatchStats)]),
         "cpx_server_create": (ctypes.c_int, [vp, P(CpxServerOpts), P(vp)]),
         "cpx_server_submit": (ctypes.c_int, [vp, CpxU128, P(U64)]),
         "cpx_server_cancel": (ctypes.c_int, [vp, U64]),
@@ -228,6 +232,7 @@ class Context:
             raise CpxError(rc, "cpx_ctx_create")
         self.h = h
         self._L = L
+        self._inflight = {}            # ticket -> batch size of submitted batches
 
     def close(self):
         if self.h:
@@ -381,6 +386,44 @@ class Context:
         if stats:
             return res, {f: getattr(st, f) for f, _ in CpxBatchStats._fields_}
         return res
+
+    def pi_batch_submit(self, xs, use_graph=True, p2_table=0):
+        """Plan and queue a batch without waiting; returns a ticket for
+        pi_batch_collect.  Two batches may be in flight per context: the next
+        one is planned and uploaded while the previous one runs."""
+        n = len(xs)
+        arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
+        bo = CpxBatchOpts(int(use_graph), 0, int(p2_table))
+        t = U64()
+        self._chk(self._L.cpx_pi_batch_submit(self.h, arr, n, ctypes.byref(bo), ctypes.byref(t)),
+                  "pi_batch_submit")
+        self._inflight[t.value] = n
+        return t.value
+
+    def pi_batch_collect(self, ticket, stats=False):
+        """Wait for a submitted batch; its pi(x) values in submission order."""
+        n = self._inflight.pop(ticket, None)     # collected (or failed) either way
+        if n is None:
+            raise CpxError(-7, "pi_batch_collect: unknown ticket")
+        out = (CpxU128 * max(n, 1))()
+        st = CpxBatchStats()
+        self._chk(self._L.cpx_pi_batch_collect(self.h, U64(ticket), out, ctypes.byref(st)),
+                  "pi_batch_collect")
+        res = [from_u128(out[i]) for i in range(n)]
+        if stats:
+            return res, {f: getattr(st, f) for f, _ in CpxBatchStats._fields_}
+        return res
+
+    def pi_batches(self, batches, use_graph=True, p2_table=0):
+        """pi(x) of every batch in an iterable of x lists, pipelined two deep:
+        yields each batch's results in order."""
+        pending = []
+        for xs in batches:
+            pending.append(self.pi_batch_submit(xs, use_graph=use_graph, p2_table=p2_table))
+            if len(pending) == 2:
+                yield self.pi_batch_collect(pending.pop(0))
+        while pending:
+            yield self.pi_batch_collect(pending.pop(0))
 
     def autotune(self, xs):
         n = len(xs)

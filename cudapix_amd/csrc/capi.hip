@@ -5,7 +5,6 @@
 #include <cstring>
 #include <random>
 
-void forget_arena(const cpx_ctx *c);
 
 static double now_ms()
 {
@@ -74,8 +73,9 @@ int cpx_ctx_create(const cpx_opts *o, cpx_ctx **out)
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
     int r = hip_fail(c, e, "hipStreamCreate"); delete c; return r;
   }
-  for (auto &ev : c->ev)
-    if ((e = hipEventCreate(&ev)) != hipSuccess) { int r = hip_fail(c, e, "hipEventCreate"); cpx_ctx_destroy(c); return r; }
+  if ((e = hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking)) != hipSuccess) {
+    int r = hip_fail(c, e, "hipStreamCreate"); cpx_ctx_destroy(c); return r;
+  }
   hipDeviceProp_t prop;
   if ((e = hipGetDeviceProperties(&prop, op.device)) != hipSuccess) {
     int r = hip_fail(c, e, "hipGetDeviceProperties"); cpx_ctx_destroy(c); return r;
@@ -99,21 +99,16 @@ int cpx_ctx_destroy(cpx_ctx *c)
 {
   if (!c) return CPX_EINVAL;
   (void)hipSetDevice(c->device);
+  if (c->up_stream) (void)hipStreamSynchronize(c->up_stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &g : c->gexec)
-    if (g) (void)hipGraphExecDestroy(g);
+  free_slots(c);
   free_tables(c);
   if (c->small.p) (void)hipFree(c->small.p);
   if (c->small_primes.p) (void)hipFree(c->small_primes.p);
   if (c->wheel.p) (void)hipFree(c->wheel.p);
   if (c->winfo.p) (void)hipFree(c->winfo.p);
-  if (c->d_plan) (void)hipFree(c->d_plan);
-  if (c->d_out) (void)hipFree(c->d_out);
-  if (c->h_plan) (void)hipHostFree(c->h_plan);
-  if (c->h_res) (void)hipHostFree(c->h_res);
-  for (auto &ev : c->ev) if (ev) (void)hipEventDestroy(ev);
+  if (c->up_stream) (void)hipStreamDestroy(c->up_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
-  forget_arena(c);
   delete c;
   return CPX_OK;
 }
@@ -129,7 +124,7 @@ int cpx_choose_y(cpx_ctx *c, cpx_u128 x, uint64_t *y)
 uint64_t cpx_ctx_device_bytes(const cpx_ctx *c)
 {
   if (!c) return 0;
-  return c->table_bytes + c->d_plan_cap + c->d_out_cap;
+  return c->table_bytes + slot_bytes(c);
 }
 
 // -------------------------------------------------------------- one query
@@ -643,6 +638,52 @@ int cpx_pi_batch_terms(cpx_ctx *c, const cpx_u128 *xs, size_t n, cpx_u128 *out, 
 {
   CTX_GUARD(c);
   return batch_impl(c, xs, n, out, terms, bo, st);
+}
+
+// ----------------------------------------------------- pipelined batches
+
+int cpx_pi_batch_submit(cpx_ctx *c, const cpx_u128 *xs, size_t n, const cpx_batch_opts *bo,
+                        uint64_t *ticket)
+{
+  CTX_GUARD(c);
+  if (!xs || !n || !ticket) return CPX_EINVAL;
+  *ticket = 0;
+  HIPCHK(hipSetDevice(c->device));
+  bool idle = true;
+  for (const BatchSlot &S : c->slot) idle = idle && !S.busy;
+  if (bo && bo->tune && !c->shape_tuned && idle) {
+    uint32_t tt, kk;
+    int rc = cpx_autotune(c, xs, n, &tt, &kk);
+    if (rc) return rc;
+  }
+  std::vector<HostQuery> hq(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = make_host_query(c, to_u128(xs[i]), 0, 0, &hq[i]);
+    if (rc) return rc;
+  }
+  uint32_t si = 0;
+  int rc = submit_queries(c, hq, TERM_ALL, bo ? bo->p2_table : 0, bo ? bo->use_graph != 0 : true, &si);
+  if (rc) return rc;
+  *ticket = c->slot[si].ticket;
+  return CPX_OK;
+}
+
+int cpx_pi_batch_collect(cpx_ctx *c, uint64_t ticket, cpx_u128 *out, cpx_batch_stats *st)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  uint32_t si = CPX_SLOTS;
+  for (uint32_t i = 0; i < CPX_SLOTS; i++)
+    if (c->slot[i].busy && c->slot[i].ticket == ticket) si = i;
+  if (si == CPX_SLOTS) return CPX_ENOTFOUND;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<ResultDev> res;
+  cpx_batch_stats local{};
+  int rc = collect_queries(c, si, res, st ? &local : nullptr);
+  if (rc) return rc;
+  for (size_t i = 0; i < res.size(); i++) out[i] = {res[i].pi[0], res[i].pi[1]};
+  if (st) *st = local;
+  return CPX_OK;
 }
 
 // ------------------------------------------------------------ autotuner

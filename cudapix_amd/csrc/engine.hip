@@ -55,6 +55,8 @@ static int dbuf_reserve(cpx_ctx *c, DBuf<T> &b, size_t n, uint64_t *acct = nullp
 {
   if (n <= b.cap && b.p) return CPX_OK;
   if (b.p) {
+    // a batch queued in another slot may still read the old table
+    (void)hipStreamSynchronize(c->stream);
     if (acct) *acct -= b.bytes();
     (void)hipFree(b.p);
     b.p = nullptr;
@@ -78,6 +80,7 @@ static void dbuf_free(DBuf<T> &b)
 
 void free_tables(cpx_ctx *c)
 {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);   // no queued batch reads them any more
   dbuf_free(c->pitab); dbuf_free(c->primes); dbuf_free(c->prinv); dbuf_free(c->mlpf);
   dbuf_free(c->scan_tot); dbuf_free(c->scan_base);
   c->pitab_nent = c->pitab_limit = c->prime_limit = c->nprimes = c->mlpf_limit = 0;
@@ -742,17 +745,6 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
 }
 
 // ------------------------------------------------------------- the arena
-// Fixed-capacity layout: array offsets depend only on capacities, so the
-// pointers baked into a captured graph stay valid until a capacity grows.
-struct Caps {
-  uint64_t q, stages, tiles, groups, ord, triv, easy, fin, p2d, tm, rows, p2rows, parts, mm, mrows;
-};
-
-struct Layout {
-  size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, p2d, rowinfo, plan_bytes;
-  size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
-};
-
 static size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static Layout make_layout(const Caps &k)
@@ -797,30 +789,11 @@ static Caps needed_caps(const Plan &pl)
   return k;
 }
 
-// per-context arena capacities live beside the buffers
-struct CtxArena { Caps caps{}; Layout L{}; };
-static std::mutex g_arena_mu;
-#include <map>
-static std::map<const cpx_ctx *, CtxArena> g_arenas;
-
-static CtxArena &arena_of(const cpx_ctx *c)
+static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
 {
-  std::lock_guard<std::mutex> lk(g_arena_mu);
-  return g_arenas[c];
-}
-
-void forget_arena(const cpx_ctx *c)
-{
-  std::lock_guard<std::mutex> lk(g_arena_mu);
-  g_arenas.erase(c);
-}
-
-static int ensure_arena(cpx_ctx *c, const Plan &pl)
-{
-  CtxArena &A = arena_of(c);
   Caps need = needed_caps(pl);
-  Caps &k = A.caps;
-  bool grow = !c->d_plan || !c->d_out;
+  Caps &k = S.caps;
+  bool grow = !S.d_plan || !S.d_out;
   auto fit = [&](uint64_t &cap, uint64_t n) {
     if (n > cap) { cap = std::max<uint64_t>(n, cap + cap / 2); grow = true; }
   };
@@ -832,31 +805,56 @@ static int ensure_arena(cpx_ctx *c, const Plan &pl)
   fit(k.mm, need.mm); fit(k.mrows, need.mrows);
   if (!grow) return CPX_OK;
   Layout L = make_layout(k);
-  if (c->d_plan) (void)hipFree(c->d_plan);
-  if (c->d_out) (void)hipFree(c->d_out);
-  if (c->h_plan) (void)hipHostFree(c->h_plan);
-  if (c->h_res) (void)hipHostFree(c->h_res);
-  c->d_plan = c->d_out = c->h_plan = c->h_res = nullptr;
-  c->generation++;
+  // this slot's last batch was collected; the other slots' buffers are untouched
+  if (S.d_plan) (void)hipFree(S.d_plan);
+  if (S.d_out) (void)hipFree(S.d_out);
+  if (S.h_plan) (void)hipHostFree(S.h_plan);
+  if (S.h_res) (void)hipHostFree(S.h_res);
+  S.d_plan = S.d_out = S.h_plan = S.h_res = nullptr;
+  S.L = Layout{};
+  S.arena_gen++;
+  const Caps kk = k;
+  auto fail = [&](int rc) { S.caps = Caps{}; return rc; };
   uint64_t budget = c->opts.hbm_budget;
-  if (budget && c->table_bytes + L.plan_bytes + L.out_bytes > budget)
-    return ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget");
+  if (budget && c->table_bytes + slot_bytes(c) + L.plan_bytes + L.out_bytes > budget)
+    return fail(ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget"));
   hipError_t e;
-  if ((e = hipMalloc(&c->d_plan, L.plan_bytes)) != hipSuccess) { c->d_plan = nullptr; (void)hipGetLastError(); A.caps = Caps{}; return hip_fail(c, e, "hipMalloc(plan)"); }
-  if ((e = hipMalloc(&c->d_out, L.out_bytes)) != hipSuccess) { c->d_out = nullptr; (void)hipGetLastError(); A.caps = Caps{}; return hip_fail(c, e, "hipMalloc(out)"); }
-  if ((e = hipHostMalloc(&c->h_plan, L.plan_bytes, hipHostMallocDefault)) != hipSuccess) { c->h_plan = nullptr; A.caps = Caps{}; return hip_fail(c, e, "hipHostMalloc"); }
-  if ((e = hipHostMalloc(&c->h_res, k.q * sizeof(ResultDev), hipHostMallocDefault)) != hipSuccess) { c->h_res = nullptr; A.caps = Caps{}; return hip_fail(c, e, "hipHostMalloc"); }
-  c->d_plan_cap = L.plan_bytes;
-  c->d_out_cap = L.out_bytes;
-  A.L = L;
+  if ((e = hipMalloc(&S.d_plan, L.plan_bytes)) != hipSuccess) { S.d_plan = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(plan)")); }
+  if ((e = hipMalloc(&S.d_out, L.out_bytes)) != hipSuccess) { S.d_out = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(out)")); }
+  if ((e = hipHostMalloc(&S.h_plan, L.plan_bytes, hipHostMallocDefault)) != hipSuccess) { S.h_plan = nullptr; return fail(hip_fail(c, e, "hipHostMalloc")); }
+  if ((e = hipHostMalloc(&S.h_res, kk.q * sizeof(ResultDev), hipHostMallocDefault)) != hipSuccess) { S.h_res = nullptr; return fail(hip_fail(c, e, "hipHostMalloc")); }
+  S.L = L;
   return CPX_OK;
 }
 
-static BatchDev make_batchdev(cpx_ctx *c)
+uint64_t slot_bytes(const cpx_ctx *c)
 {
-  CtxArena &A = arena_of(c);
-  const Layout &L = A.L;
-  char *P = (char *)c->d_plan, *O = (char *)c->d_out;
+  uint64_t b = 0;
+  for (const BatchSlot &S : c->slot)
+    if (S.d_plan && S.d_out) b += S.L.plan_bytes + S.L.out_bytes;
+  return b;
+}
+
+void free_slots(cpx_ctx *c)
+{
+  for (BatchSlot &S : c->slot) {
+    for (auto &g : S.gexec)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (S.d_plan) (void)hipFree(S.d_plan);
+    if (S.d_out) (void)hipFree(S.d_out);
+    if (S.h_plan) (void)hipHostFree(S.h_plan);
+    if (S.h_res) (void)hipHostFree(S.h_res);
+    S.d_plan = S.d_out = S.h_plan = S.h_res = nullptr;
+    for (auto &ev : S.ev)
+      if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
+    S.busy = false;
+  }
+}
+
+static BatchDev make_batchdev(cpx_ctx *c, const BatchSlot &S)
+{
+  const Layout &L = S.L;
+  char *P = (char *)S.d_plan, *O = (char *)S.d_out;
   BatchDev b{};
   b.pitab = c->pitab.p; b.pitab_n = c->pitab_nent;
   b.primes = c->primes.p; b.nprimes = c->nprimes; b.prinv = c->prinv.p;
@@ -887,11 +885,12 @@ static BatchDev make_batchdev(cpx_ctx *c)
   return b;
 }
 
-static int upload_plan(cpx_ctx *c, const Plan &pl)
+// Writes the plan into the slot's pinned staging and queues its copy on the
+// upload stream; the batch stream waits for it (event ev[4]).
+static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
 {
-  CtxArena &A = arena_of(c);
-  const Layout &L = A.L;
-  char *H = (char *)c->h_plan;
+  const Layout &L = S.L;
+  char *H = (char *)S.h_plan;
   uint32_t *cnt = (uint32_t *)(H + L.counters);
   memset(cnt, 0, 64);
   cnt[0] = (uint32_t)pl.q.size(); cnt[1] = (uint32_t)pl.tiles.size(); cnt[2] = 0;
@@ -918,7 +917,7 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   // copy only the used prefix of each array
   auto up = [&](size_t off, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
-    return hipMemcpyAsync((char *)c->d_plan + off, H + off, bytes, hipMemcpyHostToDevice, c->stream);
+    return hipMemcpyAsync((char *)S.d_plan + off, H + off, bytes, hipMemcpyHostToDevice, c->up_stream);
   };
   HIPCHK(up(L.counters, 64));
   HIPCHK(up(L.queries, pl.q.size() * sizeof(QueryDev)));
@@ -930,6 +929,8 @@ static int upload_plan(cpx_ctx *c, const Plan &pl)
   HIPCHK(up(L.p2d, pl.np2d * sizeof(BlockDesc)));
   HIPCHK(up(L.fin, pl.nfin * sizeof(BlockDesc)));
   HIPCHK(up(L.rowinfo, pl.nrows * sizeof(HardRowInfo)));
+  HIPCHK(hipEventRecord(S.ev[4], c->up_stream));
+  HIPCHK(hipStreamWaitEvent(c->stream, S.ev[4], 0));
   return CPX_OK;
 }
 
@@ -982,12 +983,12 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
   return CPX_OK;
 }
 
-static int ensure_graph(cpx_ctx *c, const BatchDev &b, uint32_t nres_words)
+static int ensure_graph(cpx_ctx *c, BatchSlot &S, const BatchDev &b, uint32_t nres_words)
 {
-  if (c->gexec[0] && c->graph_gen == c->generation && c->graph_shape.threads == c->shape.threads &&
-      c->graph_shape.kwords == c->shape.kwords)
+  if (S.gexec[0] && S.graph_gen == c->generation && S.graph_arena_gen == S.arena_gen &&
+      S.graph_shape.threads == c->shape.threads && S.graph_shape.kwords == c->shape.kwords)
     return CPX_OK;
-  for (auto &g : c->gexec)
+  for (auto &g : S.gexec)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   // kernel attributes and the occupancy queries must happen outside capture
   (void)seg_grid(c);
@@ -999,12 +1000,13 @@ static int ensure_graph(cpx_ctx *c, const BatchDev &b, uint32_t nres_words)
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
-    hipError_t e3 = hipGraphInstantiate(&c->gexec[part], g, nullptr, nullptr, 0);
+    hipError_t e3 = hipGraphInstantiate(&S.gexec[part], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e3 != hipSuccess) { c->gexec[part] = nullptr; return hip_fail(c, e3, "hipGraphInstantiate"); }
+    if (e3 != hipSuccess) { S.gexec[part] = nullptr; return hip_fail(c, e3, "hipGraphInstantiate"); }
   }
-  c->graph_gen = c->generation;
-  c->graph_shape = c->shape;
+  S.graph_gen = c->generation;
+  S.graph_arena_gen = S.arena_gen;
+  S.graph_shape = c->shape;
   return CPX_OK;
 }
 
@@ -1057,64 +1059,113 @@ static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t 
               (batch || (!c->opts.fresh_tables && c->pitab_limit >= h.zt && c->prime_limit >= h.sqrtx));
 }
 
-int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask, uint32_t p2mode,
-                std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph)
+static int submit_impl(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask, uint32_t p2mode,
+                       bool use_graph, uint32_t *slot_out)
 {
-  double t0 = now_ms();
-  if (hq_in.empty()) { res.clear(); return CPX_OK; }
+  const double t0 = now_ms();
+  uint32_t si = CPX_SLOTS;
+  for (uint32_t i = 0; i < CPX_SLOTS; i++)
+    if (!c->slot[i].busy) { si = i; break; }
+  if (si == CPX_SLOTS) return ctx_fail(c, CPX_ESTATE, "every batch slot holds an uncollected batch");
+  BatchSlot &S = c->slot[si];
+  for (auto &ev : S.ev)
+    if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDefault));
   std::vector<HostQuery> hq(hq_in);
   choose_p2tab(c, hq, mask, p2mode);
-  // tables first (planning needs the host prime list)
+  // tables first (planning needs the host prime list); a table that grows is
+  // freed only after the stream has drained (dbuf_reserve), so a batch still
+  // in flight in the other slot keeps reading valid tables
   uint64_t R = 0, P = 0, Y = 0;
   const uint64_t SB = c->shape.seg_bits();
   for (const HostQuery &h : hq) plan_needs(c, h, mask, SB, &R, &P, &Y);
-  double tt0 = now_ms();
+  const double tt0 = now_ms();
   int rc = ensure_tables(c, R, P, Y);
   if (rc) return rc;
   uint32_t cmax = 1;
   for (const HostQuery &h : hq) cmax = std::max(cmax, h.c);
   if ((rc = build_wheels(c, cmax))) return rc;
-  double tt1 = now_ms();
+  const double tt1 = now_ms();
   Plan pl;
   if ((rc = build_plan(c, hq, mask, pl))) return rc;
-  if ((rc = ensure_arena(c, pl))) return rc;
-  double tp = now_ms();
-  if ((rc = upload_plan(c, pl))) return rc;
-  BatchDev b = make_batchdev(c);
-  bool graph = use_graph && c->opts.use_graphs;
+  if ((rc = ensure_arena(c, S, pl))) return rc;
+  const double tp = now_ms();
+  if ((rc = upload_plan(c, S, pl))) return rc;
+  BatchDev b = make_batchdev(c, S);
+  const bool graph = use_graph && c->opts.use_graphs;
   // results region of the arena, in 32-bit words (capacity, so graphs stay valid)
-  const uint32_t nres_words = (uint32_t)(arena_of(c).caps.q * sizeof(ResultDev) / 4);
-  if (graph && (rc = ensure_graph(c, b, nres_words))) return rc;
-  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  const uint32_t nres_words = (uint32_t)(S.caps.q * sizeof(ResultDev) / 4);
+  if (graph && (rc = ensure_graph(c, S, b, nres_words))) return rc;
+  HIPCHK(hipEventRecord(S.ev[0], c->stream));
   for (int part = 0; part < 3; part++) {
-    if (part == 1) HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    if (graph) HIPCHK(hipGraphLaunch(c->gexec[part], c->stream));
+    if (part == 1) HIPCHK(hipEventRecord(S.ev[2], c->stream));
+    if (graph) HIPCHK(hipGraphLaunch(S.gexec[part], c->stream));
     else if ((rc = enqueue_part(c, b, part, nres_words))) return rc;
-    if (part == 1) HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    if (part == 1) HIPCHK(hipEventRecord(S.ev[3], c->stream));
   }
-  HIPCHK(hipEventRecord(c->ev[1], c->stream));
-  CtxArena &A = arena_of(c);
-  HIPCHK(hipMemcpyAsync(c->h_res, (char *)c->d_out + A.L.results, hq.size() * sizeof(ResultDev),
+  HIPCHK(hipMemcpyAsync(S.h_res, (char *)S.d_out + S.L.results, hq.size() * sizeof(ResultDev),
                         hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  res.assign((ResultDev *)c->h_res, (ResultDev *)c->h_res + hq.size());
+  HIPCHK(hipEventRecord(S.ev[1], c->stream));
+  S.busy = true;
+  S.ticket = c->next_ticket++;
+  S.nq = hq.size();
+  S.t_submit = t0;
+  S.ms_tables = tt1 - tt0;
+  S.ms_plan = (tp - t0) - (tt1 - tt0);
+  S.tiles = pl.tiles.size();
+  S.hard_segments = pl.hard_segments;
+  S.p2_segments = pl.p2_segments;
+  *slot_out = si;
+  return CPX_OK;
+}
+
+int submit_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
+                   bool use_graph, uint32_t *slot)
+{
+  if (hq.empty()) return ctx_fail(c, CPX_EINVAL, "empty batch");
+  const int rc = submit_impl(c, hq, mask, p2mode, use_graph, slot);
+  if (rc) {
+    // a failed submission may have queued part of its upload or launches:
+    // let them drain before the slot is reused
+    (void)hipStreamSynchronize(c->up_stream);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  return rc;
+}
+
+int collect_queries(cpx_ctx *c, uint32_t si, std::vector<ResultDev> &res, cpx_batch_stats *st)
+{
+  if (si >= CPX_SLOTS || !c->slot[si].busy) return ctx_fail(c, CPX_ESTATE, "no batch in this slot");
+  BatchSlot &S = c->slot[si];
+  S.busy = false;
+  HIPCHK(hipEventSynchronize(S.ev[1]));
+  res.assign((ResultDev *)S.h_res, (ResultDev *)S.h_res + S.nq);
   for (const ResultDev &r : res)
     if (r.status != 1) return ctx_fail(c, CPX_EDEVICE, "a query produced no result");
   if (st) {
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms, S.ev[0], S.ev[1]);
     st->ms_device = ms;
     float s2 = 0;
-    (void)hipEventElapsedTime(&s2, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&s2, S.ev[2], S.ev[3]);
     st->ms_segments = s2;
-    st->ms_tables = tt1 - tt0;
-    st->ms_plan = (tp - t0) - (tt1 - tt0);
-    st->ms_total = now_ms() - t0;
-    st->tiles = pl.tiles.size();
-    st->hard_segments = pl.hard_segments;
-    st->p2_segments = pl.p2_segments;
+    st->ms_tables = S.ms_tables;
+    st->ms_plan = S.ms_plan;
+    st->ms_total = now_ms() - S.t_submit;
+    st->tiles = S.tiles;
+    st->hard_segments = S.hard_segments;
+    st->p2_segments = S.p2_segments;
     st->seg_threads = (uint32_t)c->shape.threads;
     st->seg_kwords = (uint32_t)c->shape.kwords;
   }
   return CPX_OK;
+}
+
+int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
+                std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph)
+{
+  if (hq.empty()) { res.clear(); return CPX_OK; }
+  uint32_t si = 0;
+  const int rc = submit_queries(c, hq, mask, p2mode, use_graph, &si);
+  if (rc) return rc;
+  return collect_queries(c, si, res, st);
 }

@@ -34,9 +34,48 @@ enum : uint32_t {
 };
 
 
+// Fixed-capacity batch arena: array offsets depend only on capacities, so
+// the pointers baked into a captured graph stay valid until a capacity grows.
+struct Caps {
+  uint64_t q, stages, tiles, groups, ord, triv, easy, fin, p2d, tm, rows, p2rows, parts, mm, mrows;
+};
+
+struct Layout {
+  size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, p2d, rowinfo, plan_bytes;
+  size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
+};
+
+// Batch slots per context: one batch can be planned and uploaded while the
+// previous one runs on the device.
+#define CPX_SLOTS 2
+
+// One batch's device arena, pinned staging, captured launch graphs and
+// events, and the record of the batch it holds until collected.
+struct BatchSlot {
+  void *d_plan = nullptr;           // plan arena (device)
+  void *h_plan = nullptr;           // its pinned staging copy
+  void *d_out = nullptr;            // counters, partial sums and results (device)
+  void *h_res = nullptr;            // pinned results
+  Caps caps{};
+  Layout L{};
+  uint64_t arena_gen = 0;           // bumps when the arena is reallocated
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};   // pre-terms, segments, post
+  uint64_t graph_gen = ~0ull, graph_arena_gen = ~0ull;     // table and arena generations captured
+  SegShape graph_shape{0, 0};
+  // batch start, batch end, segment kernel start and end, upload done
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // the batch in flight
+  bool busy = false;
+  uint64_t ticket = 0;
+  size_t nq = 0;
+  double t_submit = 0, ms_tables = 0, ms_plan = 0;
+  uint64_t tiles = 0, hard_segments = 0, p2_segments = 0;
+};
+
 struct cpx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;     // tables and batch kernels
+  hipStream_t up_stream = nullptr;  // plan uploads (overlap the previous batch's kernels)
   cpx_opts opts{};
   std::string err;
   int cu_count = 0;
@@ -62,12 +101,10 @@ struct cpx_ctx {
   DBuf<uint64_t> scan_base;
   uint64_t table_bytes = 0;
 
-  // ---- batch buffers (plan arena on the device, pinned staging on the host)
-  void *d_plan = nullptr;  size_t d_plan_cap = 0;
-  void *h_plan = nullptr;  size_t h_plan_cap = 0;
-  void *d_out = nullptr;   size_t d_out_cap = 0;
-  void *h_res = nullptr;   size_t h_res_cap = 0;
-  uint64_t generation = 0;                      // bumps on any reallocation
+  // ---- batch slots
+  BatchSlot slot[CPX_SLOTS];
+  uint64_t next_ticket = 1;
+  uint64_t generation = 0;                      // bumps when a table is reallocated
 
   // segment-kernel shape: 512 threads x 32 words (2^19-bit segments, two
   // blocks per CU) measured fastest for single queries to 1e16 on MI355X;
@@ -75,13 +112,6 @@ struct cpx_ctx {
   SegShape shape{512, 32};
   bool shape_tuned = false;
   double alpha_scale = 1.0;                     // factor on the alpha model (autotuned)
-
-  // captured graph of the batch launch sequence
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};   // pre-terms, segments, post
-  uint64_t graph_gen = ~0ull;
-  SegShape graph_shape{0, 0};
-
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 // engine.hip
@@ -90,9 +120,19 @@ int hip_fail(cpx_ctx *c, hipError_t e, const char *what);
 uint64_t choose_y_host(const cpx_ctx *c, u128 x);
 int make_host_query(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, HostQuery *hq);
 int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y);
-// p2mode: cpx_batch_opts.p2_table
+// p2mode: cpx_batch_opts.p2_table.  submit_queries plans the queries into a
+// free slot and queues their upload, launches and result copy without
+// waiting (CPX_ESTATE when every slot holds an uncollected batch);
+// collect_queries waits for a slot's batch and frees the slot; run_queries
+// does both.
+int submit_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
+                   bool use_graph, uint32_t *slot);
+int collect_queries(cpx_ctx *c, uint32_t slot, std::vector<ResultDev> &res, cpx_batch_stats *st);
 int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
                 std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph);
+// Frees every slot's buffers, graphs and events (context teardown).
+void free_slots(cpx_ctx *c);
+uint64_t slot_bytes(const cpx_ctx *c);
 uint64_t host_pi(const cpx_ctx *c, uint64_t v);   // v <= prime_limit
 uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &hq);
 void free_tables(cpx_ctx *c);

@@ -175,6 +175,19 @@ int cpx_pi_batch(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
 /* Same, also returning each query's term breakdown (terms may be NULL). */
 int cpx_pi_batch_terms(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
                        cpx_terms *terms, const cpx_batch_opts *opts, cpx_batch_stats *stats);
+/* Pipelined batches (no reference counterpart: the reference solves one x per
+ * run).  cpx_pi_batch_submit plans the batch on the calling thread, queues its
+ * upload and launches and returns a ticket without waiting, so the next batch
+ * is planned and uploaded while this one runs; cpx_pi_batch_collect waits for
+ * a ticket's batch and writes its n results to out (stats as cpx_pi_batch,
+ * ms_total from submit to collect).  A context holds two uncollected batches
+ * at most: a further submit (or a cpx_pi_batch / cpx_pi call while both are
+ * in flight) returns CPX_ESTATE.  Tickets may be collected in any order; an
+ * unknown or collected ticket gives CPX_ENOTFOUND.  opts->tune is honoured
+ * only when no batch is in flight. */
+int cpx_pi_batch_submit(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, const cpx_batch_opts *opts,
+                        uint64_t *ticket);
+int cpx_pi_batch_collect(cpx_ctx *ctx, uint64_t ticket, cpx_u128 *out, cpx_batch_stats *stats);
 
 /* ---- autotuner ----------------------------------------------------------- */
 /* Time a calibration batch under each candidate and keep the fastest for this

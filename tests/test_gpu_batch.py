@@ -49,6 +49,46 @@ def test_batch_terms_match_oracle(ctx, oracle):
             assert t[k] == ref[k], (x, k)
 
 
+def test_pipelined_batches(oracle):
+    """Two batches in flight: each collected result equals the batch solved
+    with one synchronous call; tickets collect in any order; a third
+    submission, or a synchronous batch while both slots are busy, is refused
+    with CPX_ESTATE; an unknown ticket gives CPX_ENOTFOUND."""
+    import cudapix_amd as C
+    with C.Context(device=0) as ctx:     # its own slots: a failure leaves no batch in flight elsewhere
+        _pipelined(C, ctx, oracle)
+
+
+def _pipelined(C, ctx, oracle):
+    xs = c3()
+    b1, b2, b3 = sorted(xs[:40]), xs[40:100], [10 ** 9, 2 ** 32, 16384, 5] + xs[100:120]
+    want = [ctx.pi_batch(b) for b in (b1, b2, b3)]
+    t1 = ctx.pi_batch_submit(b1)
+    t2 = ctx.pi_batch_submit(b2)
+    with pytest.raises(C.CpxError) as e:
+        ctx.pi_batch_submit(b3)
+    assert e.value.code == -8
+    with pytest.raises(C.CpxError) as e:
+        ctx.pi_batch(b3)
+    assert e.value.code == -8
+    out = (C.CpxU128 * 4)()
+    assert ctx._L.cpx_pi_batch_collect(ctx.h, C.U64(t2 + 1000), out, None) == -7
+    assert ctx.pi_batch_collect(t2) == want[1]          # out of order
+    t3 = ctx.pi_batch_submit(b3)
+    r1, st = ctx.pi_batch_collect(t1, stats=True)
+    assert r1 == want[0] and st["ms_device"] > 0 and st["tiles"] > 0
+    assert ctx.pi_batch_collect(t3) == want[2]
+    assert ctx._L.cpx_pi_batch_collect(ctx.h, C.U64(t3), out, None) == -7   # collected already
+    for x, v in zip(b3, want[2]):
+        if x <= 3 * 10 ** 9:
+            assert v == oracle.count_primes(0, x + 1), x
+    # a stream of batches of varied sizes, pipelined two deep, also after a
+    # larger batch grew the slots
+    batches = [b2, b1[:3], b3, b1, b2[:1]]
+    assert list(ctx.pi_batches(batches)) == [ctx.pi_batch(b) for b in batches]
+    assert list(ctx.pi_batches(batches, use_graph=False)) == [ctx.pi_batch(b) for b in batches]
+
+
 def test_p2_lookup_equals_p2_segments():
     """P2 from lookups in the resident pi table (a batch of >= 8 queries)
     equals P2 counted in the sieve's P2 segments (p2_table=2), term by term,
