@@ -5,7 +5,10 @@ A step is one batched solve of the 512-query workload (x log-uniform in
 [1e9, 1e15], splitmix64 seed 3, tests/golden/queries_c3.json) through the
 C ABI: every hard-leaf and P2 segment of every query is a tile of one
 persistent kernel launch over shared resident tables, replayed from captured
-hipGraphs.  value = 512 * steps / timed seconds (tables already resident).
+hipGraphs.  The steps are pipelined two deep (cpx_pi_batch_submit/collect:
+a step is planned and uploaded while the previous one runs); value = 512 *
+steps / timed seconds (tables already resident).  sync_qps is the same
+steps made one synchronous call at a time.
 
 Beside it, in the same run:
   * plain: the same 512 x solved one at a time, each with its own table build
@@ -171,6 +174,61 @@ def cpu_baseline(xs, budget_s=20.0):
                       % (done, t_sum, "" if done == k else ", untimed strata extrapolated x2")}
 
 
+def configs_run(args):
+    """The single-query configurations as markdown rows on stdout: pi(1e12) at
+    alpha 2; pi(1e16) over alpha in {1, 1.5, 2, 3, 4, 6, 8} and the segment
+    sizes 2^18, 2^19, 2^20 bits (2^21 bits do not fit one block's LDS);
+    pi(1e20) at alpha 2, 4 and the built-in model.  y = floor(alpha *
+    icbrt(x)); every value is checked against the published pi(x); wall time
+    of a call on a warm context (tables resident), best of --reps."""
+    import cudapix_amd as C
+    known = load_json("known.json")["pi10"]
+
+    def icbrt(x):
+        r = int(round(x ** (1.0 / 3.0)))
+        while r ** 3 > x:
+            r -= 1
+        while (r + 1) ** 3 <= x:
+            r += 1
+        return r
+
+    def timed(cx, x, y, reps):
+        t = cx.pi(x, y=y, terms=True)               # warm-up: tables
+        best = None
+        for _ in range(reps):
+            t0 = time.time()
+            v = cx.pi(x, y=y)
+            dt = (time.time() - t0) * 1000
+            best = dt if best is None else min(best, dt)
+        if v != known[str(round(math.log10(x)))]:
+            raise SystemExit("pi(%d) wrong at y=%d: %d" % (x, y, v))
+        return best, t
+
+    print("| case | x | alpha | y | segment bits (threads x words) | ms | pi(x) |", flush=True)
+    print("|---|---|---|---|---|---|---|", flush=True)
+    with C.Context(device=0) as cx:
+        x = 10 ** 12
+        y = 2 * icbrt(x)
+        ms, t = timed(cx, x, y, args.reps)
+        print("| C1 | 1e12 | 2 | %d | 2^19 (512 x 32) | %.2f | %d |" % (y, ms, t["pi"]), flush=True)
+    for shape in ((256, 32), (512, 32), (512, 64), (256, 128)):
+        bits = 32 * shape[0] * shape[1]
+        with C.Context(device=0, seg_threads=shape[0], seg_kwords=shape[1]) as cx:
+            x = 10 ** 16
+            for alpha in (1, 1.5, 2, 3, 4, 6, 8):
+                y = int(alpha * icbrt(x))
+                ms, t = timed(cx, x, y, args.reps)
+                print("| C2 | 1e16 | %g | %d | 2^%d (%d x %d) | %.2f | %d |"
+                      % (alpha, y, round(math.log2(bits)), shape[0], shape[1], ms, t["pi"]), flush=True)
+    with C.Context(device=0) as cx:
+        x = 10 ** 20
+        for alpha in (2, 4, 0):
+            y = int(alpha * icbrt(x)) if alpha else 0
+            ms, t = timed(cx, x, y, 1)
+            print("| C5 | 1e20 | %s | %d | 2^19 (512 x 32) | %.1f | %d |"
+                  % (alpha if alpha else "model (%.1f)" % t["alpha"], t["y"], ms, t["pi"]), flush=True)
+
+
 def profile_run(args):
     """One workload only, for profiler runs: `single` = pi(x) on a warm
     context (default shape), `batch` = the 512-query batched step (with
@@ -236,6 +294,9 @@ def main():
     ap.add_argument("--x", type=float, default=1e16, help="x of --profile single")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="bytes of --profile copy")
+    ap.add_argument("--configs", action="store_true",
+                    help="time the single-query configurations (1e12, the 1e16 alpha x segment sweep, "
+                         "1e20 at two alphas) as markdown rows and exit")
     ap.add_argument("--tuning", default="",
                     help="threads,kwords,alpha_scale,c for --profile batch (as the timed run's autotune chose)")
     args = ap.parse_args()
@@ -249,6 +310,9 @@ def main():
 
     if args.profile:
         profile_run(args)
+        return
+    if args.configs:
+        configs_run(args)
         return
 
     import cudapix_amd as C
