@@ -95,6 +95,7 @@ EXPORTS = [
     "cpx_get_tuning",
     "cpx_set_tuning",
     "cpx_pi_batch_submit", "cpx_pi_batch_collect",
+    "cpx_get_device_primes32", "cpx_get_host_primes", "cpx_gen_device_bit_sieve",
 ]
 
 ERRORS = {0: "CPX_OK", -1: "CPX_EINVAL", -2: "CPX_ENOMEM", -3: "CPX_EDEVICE", -4: "CPX_ERANGE",
@@ -144,6 +145,10 @@ def lib():
         "cpx_v_split_u64": (ctypes.c_int, [vp, U64, U64, P(ctypes.c_int64), P(ctypes.c_int64)]),
         "cpx_count_primes": (ctypes.c_int, [vp, U64, U64, P(U64)]),
         "cpx_get_device_primes": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
+        "cpx_get_device_primes32": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
+        "cpx_get_host_primes": (ctypes.c_int, [vp, U64, U64, P(U64), ctypes.c_size_t,
+                                               P(ctypes.c_size_t)]),
+        "cpx_gen_device_bit_sieve": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
         "cpx_get_d_pitable": (ctypes.c_int, [vp, ctypes.c_uint32, P(vp)]),
         "cpx_pitable_create": (ctypes.c_int, [vp, U64, U64, U64, P(vp)]),
         "cpx_pitable_destroy": (ctypes.c_int, [vp]),
@@ -323,6 +328,37 @@ class Context:
                   "getDevicePrimes")
         try:
             return self._d2h(d, n.value, U64)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def getDevicePrimes32(self, lo, hi):
+        d = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        self._chk(self._L.cpx_get_device_primes32(self.h, lo, hi, ctypes.byref(d), ctypes.byref(n)),
+                  "getDevicePrimes32")
+        try:
+            return self._d2h(d, n.value, ctypes.c_uint32)
+        finally:
+            self._L.cpx_free(self.h, d)
+
+    def getHostPrimes(self, lo, hi):
+        n = ctypes.c_size_t()
+        self._chk(self._L.cpx_get_host_primes(self.h, lo, hi, None, 0, ctypes.byref(n)), "getHostPrimes")
+        buf = (U64 * max(n.value, 1))()
+        m = ctypes.c_size_t()
+        self._chk(self._L.cpx_get_host_primes(self.h, lo, hi, buf, n.value, ctypes.byref(m)),
+                  "getHostPrimes")
+        return list(buf[:m.value])
+
+    def genDeviceBitSieve(self, lo, hi):
+        """Odd-only bit sieve words of [lo, hi) (lo a multiple of 64): bit i of
+        word j is clear iff lo + 64 j + 2 i + 1 is an odd prime in [lo, hi)."""
+        d = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        self._chk(self._L.cpx_gen_device_bit_sieve(self.h, lo, hi, ctypes.byref(d), ctypes.byref(n)),
+                  "genDeviceBitSieve")
+        try:
+            return self._d2h(d, n.value, ctypes.c_uint32)
         finally:
             self._L.cpx_free(self.h, d)
 

@@ -331,8 +331,10 @@ int cpx_count_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t *out)
   HIPCHK(hipMalloc(&d, sizeof(unsigned long long)));
   int rc = CPX_OK;
   unsigned long long h = 0;
+  // decided once: the sieve branch builds tables that may then reach hi
+  const bool from_table = hi - 1 <= c->pitab_limit && c->pitab.p;
   do {
-    if (hi - 1 <= c->pitab_limit && c->pitab.p) {
+    if (from_table) {
       hipError_t e = launch_count_range(c->pitab.p, lo, hi, d, c->stream);
       if (e) { rc = hip_fail(c, e, "count_range"); break; }
     } else {
@@ -346,7 +348,7 @@ int cpx_count_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t *out)
     if (!e) e = hipStreamSynchronize(c->stream);
     if (e) { rc = hip_fail(c, e, "count copy"); break; }
     // the sieve counts odd primes >= 3; add the prime 2
-    if (!(hi - 1 <= c->pitab_limit && c->pitab.p) && lo <= 2 && hi > 2) h += 1;
+    if (!from_table && lo <= 2 && hi > 2) h += 1;
   } while (0);
   (void)hipFree(d);
   if (!rc) *out = h;
@@ -576,12 +578,64 @@ int cpx_gen_d_mu(cpx_ctx *c, uint64_t bottom, uint64_t top, int8_t **d_out)
   return CPX_OK;
 }
 
-int cpx_get_device_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len)
+// Ranges above this would let a sieve segment's end (2^20 past hi) or the
+// sieving-prime limit sqrt(hi) leave 64 and 32 bits.
+static const uint64_t RANGE_HI_MAX = ~0ull - (1ull << 34);
+// Below this hi the primes come from the pi table, grown to hi if needed.
+static const uint64_t RANGE_TABLE_MAX = 1ull << 24;
+
+// Primes of [lo, hi) into a new device array (u64, or u32 when out32) by the
+// segmented bit sieve: sieving primes to sqrt(hi) only, one 2^20-integer
+// segment per block, per-segment counts scanned on the host, then each
+// segment's primes written at its offset.
+static int range_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, bool out32, void **d_out, size_t *len)
 {
-  CTX_GUARD(c);
-  if (!d_out || !len) return CPX_EINVAL;
-  *d_out = nullptr; *len = 0;
-  if (hi <= lo) return CPX_OK;
+  if (hi > RANGE_HI_MAX) return ctx_fail(c, CPX_ERANGE, "prime range above 2^64 - 2^34");
+  const uint64_t base = lo & ~63ull;
+  const uint64_t nseg = range_sieve_segments(base, hi);
+  const uint64_t nwords = (hi - base + 63) / 64;
+  const uint64_t r = cpx_isqrt128((u128)hi + (1ull << 21)) + 1;     // covers the last segment's end
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  uint32_t *words = nullptr, *cnt = nullptr;
+  uint64_t *off = nullptr;
+  void *out = nullptr;
+  const size_t esz = out32 ? 4 : 8;
+  hipError_t e = hipMalloc(&words, nwords * 4);
+  if (!e) e = hipMalloc(&cnt, nseg * 4);
+  if (!e) e = hipMalloc(&off, nseg * 8);
+  if (!e) e = launch_bit_sieve(base, lo, hi, nwords, c->primes.p, (uint32_t)c->nprimes, false, words, cnt,
+                               c->stream);
+  std::vector<uint32_t> hc(nseg);
+  std::vector<uint64_t> ho(nseg);
+  if (!e) e = hipMemcpyAsync(hc.data(), cnt, nseg * 4, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  uint64_t total = 0;
+  for (uint64_t b = 0; b < nseg; b++) { ho[b] = total; total += hc[b]; }
+  const bool has2 = lo <= 2 && hi > 2;
+  const uint64_t n = total + (has2 ? 1 : 0);
+  if (!e) e = hipMalloc(&out, std::max<uint64_t>(n, 1) * esz);
+  if (!e) e = hipMemcpyAsync(off, ho.data(), nseg * 8, hipMemcpyHostToDevice, c->stream);
+  if (!e) e = launch_bit_extract(base, hi, words, nwords, off, (char *)out + (has2 ? esz : 0), out32, c->stream);
+  const uint64_t two = 2;
+  if (!e && has2) e = hipMemcpyAsync(out, &two, esz, hipMemcpyHostToDevice, c->stream);   // little-endian
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (words) (void)hipFree(words);
+  if (cnt) (void)hipFree(cnt);
+  if (off) (void)hipFree(off);
+  if (e) {
+    if (out) (void)hipFree(out);
+    (void)hipGetLastError();
+    return hip_fail(c, e, "prime range sieve");
+  }
+  *d_out = out;
+  *len = n;
+  return CPX_OK;
+}
+
+// Primes of [lo, hi) from the pi table (which then reaches hi).
+static int table_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len)
+{
   int rc = ensure_tables(c, hi + 1, 64, 64);
   if (rc) return rc;
   uint64_t cnt = 0;
@@ -593,6 +647,86 @@ int cpx_get_device_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out
   if (e) { (void)hipFree(d); return hip_fail(c, e, "getDevicePrimes"); }
   *d_out = d;
   *len = cnt;
+  return CPX_OK;
+}
+
+int cpx_get_device_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len)
+{
+  CTX_GUARD(c);
+  if (!d_out || !len) return CPX_EINVAL;
+  *d_out = nullptr; *len = 0;
+  if (hi <= lo) return CPX_OK;
+  HIPCHK(hipSetDevice(c->device));
+  if (hi <= RANGE_TABLE_MAX || (c->pitab.p && hi - 1 <= c->pitab_limit))
+    return table_primes(c, lo, hi, d_out, len);
+  void *d = nullptr;
+  int rc = range_primes(c, lo, hi, false, &d, len);
+  if (!rc) *d_out = (uint64_t *)d;
+  return rc;
+}
+
+int cpx_get_device_primes32(cpx_ctx *c, uint64_t lo, uint64_t hi, uint32_t **d_out, size_t *len)
+{
+  CTX_GUARD(c);
+  if (!d_out || !len) return CPX_EINVAL;
+  *d_out = nullptr; *len = 0;
+  if (hi > (1ull << 32)) return ctx_fail(c, CPX_ERANGE, "getDevicePrimes32: primes above 2^32");
+  if (hi <= lo) return CPX_OK;
+  HIPCHK(hipSetDevice(c->device));
+  void *d = nullptr;
+  int rc = range_primes(c, lo, hi, true, &d, len);
+  if (!rc) *d_out = (uint32_t *)d;
+  return rc;
+}
+
+int cpx_get_host_primes(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t *out, size_t cap, size_t *len)
+{
+  CTX_GUARD(c);
+  if (!len) return CPX_EINVAL;
+  *len = 0;
+  if (hi <= lo) return CPX_OK;
+  if (!out) {
+    uint64_t n = 0;
+    int rc = cpx_count_primes(c, lo, hi, &n);
+    if (!rc) *len = n;
+    return rc;
+  }
+  uint64_t *d = nullptr;
+  size_t n = 0;
+  int rc = cpx_get_device_primes(c, lo, hi, &d, &n);
+  if (rc) return rc;
+  *len = n;
+  hipError_t e = n <= cap ? hipMemcpy(out, d, n * 8, hipMemcpyDeviceToHost) : hipSuccess;
+  (void)hipFree(d);
+  if (e) return hip_fail(c, e, "getHostPrimes copy");
+  return n <= cap ? CPX_OK : ctx_fail(c, CPX_EINVAL, "getHostPrimes: output buffer too small");
+}
+
+int cpx_gen_device_bit_sieve(cpx_ctx *c, uint64_t lo, uint64_t hi, uint32_t **d_words, size_t *nwords)
+{
+  CTX_GUARD(c);
+  if (!d_words || !nwords || (lo & 63)) return CPX_EINVAL;
+  *d_words = nullptr; *nwords = 0;
+  if (hi <= lo) return CPX_OK;
+  if (hi > RANGE_HI_MAX) return ctx_fail(c, CPX_ERANGE, "bit sieve above 2^64 - 2^34");
+  HIPCHK(hipSetDevice(c->device));
+  const uint64_t nw = (hi - lo + 63) / 64, nseg = range_sieve_segments(lo, hi);
+  const uint64_t r = cpx_isqrt128((u128)hi + (1ull << 21)) + 1;
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  uint32_t *w = nullptr, *cnt = nullptr;
+  hipError_t e = hipMalloc(&w, nw * 4);
+  if (!e) e = hipMalloc(&cnt, nseg * 4);
+  if (!e) e = launch_bit_sieve(lo, lo, hi, nw, c->primes.p, (uint32_t)c->nprimes, true, w, cnt, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (cnt) (void)hipFree(cnt);
+  if (e) {
+    if (w) (void)hipFree(w);
+    (void)hipGetLastError();
+    return hip_fail(c, e, "genDeviceBitSieve");
+  }
+  *d_words = w;
+  *nwords = nw;
   return CPX_OK;
 }
 

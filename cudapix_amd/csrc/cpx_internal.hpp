@@ -231,5 +231,14 @@ hipError_t launch_mu_lpf_range(uint32_t *lpf, int8_t *mu, uint64_t ibase, uint64
                                const uint32_t *primes, uint32_t nsp, hipStream_t s);
 hipError_t launch_phi_prefix(const uint32_t *lpf, uint64_t n, uint32_t pa, uint32_t *out,
                              hipStream_t s);
+// odd-only prime bit sieve of [base, hi) (base a multiple of 64) in
+// segments of 2^20 integers, with per-segment prime counts; extraction of
+// the primes from such words (prime = 1) at per-segment output offsets
+uint64_t range_sieve_segments(uint64_t base, uint64_t hi);
+hipError_t launch_bit_sieve(uint64_t base, uint64_t lo, uint64_t hi, uint64_t nwords, const uint32_t *sp,
+                            uint32_t nsp, bool composite_ones, uint32_t *words, uint32_t *seg_cnt,
+                            hipStream_t s);
+hipError_t launch_bit_extract(uint64_t base, uint64_t hi, const uint32_t *words, uint64_t nwords,
+                              const uint64_t *seg_off, void *out, bool out32, hipStream_t s);
 hipError_t launch_extract_range_u64(const PiEntry *tab, uint64_t lo, uint64_t hi, uint64_t *out,
                                     uint64_t cap, hipStream_t s);

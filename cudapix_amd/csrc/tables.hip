@@ -451,6 +451,127 @@ hipError_t launch_count_primes_sieve(uint64_t lo, uint64_t hi, const uint32_t *s
   return hipGetLastError();
 }
 
+// ------------------------------------------- odd-only prime bit sieve
+// Segmented Eratosthenes over words of 64 integers from `base` (a multiple
+// of 64): bit i of word j stands for n = base + 64 j + 2 i + 1.  A block
+// sieves one segment of RS_K * RS_T words in LDS; word j gets 1 for an odd
+// prime n in [lo, hi) (n >= 3) and 0 otherwise, or the complement when
+// `composite_ones` (the genDeviceBitSieve convention the reference's hard
+// leaves read: a clear bit is a prime, src/sieve/S2_hard_device.cu:248-249).
+// seg_cnt[b] = primes in segment b.
+#define RS_T 256
+#define RS_K 64                                    // 2^20 integers per segment
+
+__global__ __launch_bounds__(RS_T) void k_bit_sieve(uint64_t base, uint64_t lo, uint64_t hi,
+                                                    uint64_t nwords, const uint32_t *__restrict__ sp,
+                                                    uint32_t nsp, uint32_t composite_ones,
+                                                    uint32_t *words, uint32_t *seg_cnt)
+{
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t wsum[RS_T / CPX_WAVE];
+  const uint32_t t = threadIdx.x;
+  constexpr uint32_t NWS = RS_K * RS_T;            // words per segment
+  const uint64_t s0 = base + (uint64_t)blockIdx.x * 64 * NWS;
+  for (uint32_t g = t; g < NWS; g += RS_T) lds[seg_addr(g, RS_K, RS_T)] = 0xffffffffu;
+  __syncthreads();
+  seg_cross_primes(lds, RS_K, RS_T, s0, 32ull * NWS, sp, 1, nsp);
+  __syncthreads();
+  uint32_t c = 0;
+  for (uint32_t g = t; g < NWS; g += RS_T) {       // coalesced word order
+    const uint64_t gw = (uint64_t)blockIdx.x * NWS + g;
+    if (gw >= nwords) break;
+    uint32_t w = lds[seg_addr(g, RS_K, RS_T)];
+    const uint64_t n0 = s0 + 64 * (uint64_t)g + 1;     // integer of bit 0
+    if (n0 < lo || n0 + 62 >= hi || n0 < 3) {
+      uint32_t m = 0;
+      for (uint32_t i = 0; i < 32; i++) {
+        const uint64_t n = n0 + 2 * i;
+        m |= (n >= lo && n < hi && n >= 3) ? (1u << i) : 0u;
+      }
+      w &= m;
+    }
+    c += cpx_popc32(w);
+    if (words) words[gw] = composite_ones ? ~w : w;
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0) wsum[wave_id()] = c;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < RS_T / CPX_WAVE; w++) tot += wsum[w];
+    seg_cnt[blockIdx.x] = tot;
+  }
+}
+
+// The primes of each segment's words (prime = 1), in order, from out index
+// seg_off[b] (the exclusive scan of the segment counts): thread t of block b
+// takes words [t*RS_K, (t+1)*RS_K) of the segment, after a block scan of
+// their popcounts.
+template <typename OutT>
+__global__ __launch_bounds__(RS_T) void k_bit_extract(uint64_t base, const uint32_t *__restrict__ words,
+                                                      uint64_t nwords, const uint64_t *__restrict__ seg_off,
+                                                      OutT *out)
+{
+  __shared__ uint32_t wtot[RS_T / CPX_WAVE];
+  const uint32_t t = threadIdx.x;
+  constexpr uint32_t NWS = RS_K * RS_T;
+  const uint64_t w0 = (uint64_t)blockIdx.x * NWS + (uint64_t)t * RS_K;
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < RS_K; j++)
+    if (w0 + j < nwords) c += cpx_popc32(words[w0 + j]);
+  const uint32_t incl = wave_incl_scan(c);
+  if (lane_id() == CPX_WAVE - 1) wtot[wave_id()] = incl;
+  __syncthreads();
+  uint64_t k = seg_off[blockIdx.x] + (incl - c);
+  for (uint32_t w = 0; w < wave_id(); w++) k += wtot[w];
+  for (uint32_t j = 0; j < RS_K; j++) {
+    if (w0 + j >= nwords) break;
+    uint32_t b = words[w0 + j];
+    while (b) {
+      const uint32_t i = __builtin_ctz(b);
+      b &= b - 1;
+      out[k++] = (OutT)(base + 64 * (w0 + j) + 2 * i + 1);
+    }
+  }
+}
+
+uint64_t range_sieve_segments(uint64_t base, uint64_t hi)
+{
+  const uint64_t span = 64ull * RS_K * RS_T;
+  return (hi - base + span - 1) / span;
+}
+
+size_t range_sieve_lds() { return (size_t)(RS_K + 1) * RS_T * 4; }
+
+hipError_t launch_bit_sieve(uint64_t base, uint64_t lo, uint64_t hi, uint64_t nwords, const uint32_t *sp,
+                            uint32_t nsp, bool composite_ones, uint32_t *words, uint32_t *seg_cnt,
+                            hipStream_t s)
+{
+  const uint64_t nseg = range_sieve_segments(base, hi);
+  if (nseg == 0) return hipSuccess;
+  if (nseg > 0x7fffffffull) return hipErrorInvalidValue;
+  (void)hipFuncSetAttribute((const void *)k_bit_sieve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)range_sieve_lds());
+  hipLaunchKernelGGL(k_bit_sieve, dim3((uint32_t)nseg), dim3(RS_T), range_sieve_lds(), s, base, lo, hi,
+                     nwords, sp, nsp, composite_ones ? 1u : 0u, words, seg_cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_bit_extract(uint64_t base, uint64_t hi, const uint32_t *words, uint64_t nwords,
+                              const uint64_t *seg_off, void *out, bool out32, hipStream_t s)
+{
+  const uint64_t nseg = range_sieve_segments(base, hi);
+  if (nseg == 0) return hipSuccess;
+  if (nseg > 0x7fffffffull) return hipErrorInvalidValue;
+  if (out32)
+    hipLaunchKernelGGL(k_bit_extract<uint32_t>, dim3((uint32_t)nseg), dim3(RS_T), 0, s, base, words, nwords,
+                       seg_off, (uint32_t *)out);
+  else
+    hipLaunchKernelGGL(k_bit_extract<uint64_t>, dim3((uint32_t)nseg), dim3(RS_T), 0, s, base, words, nwords,
+                       seg_off, (uint64_t *)out);
+  return hipGetLastError();
+}
+
 // x / y[i] for a 128-bit x, in place (divXbyY, src/P2.cu:135-138).
 __global__ __launch_bounds__(256) void k_div_x_by_y(uint64_t xlo, uint64_t xhi, uint64_t *d,
                                                     size_t len)

@@ -124,6 +124,24 @@ int cpx_count_primes(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t *out);
 /* CudaSieve::getDevicePrimes(bottom, top, len): device array of the primes in
  * [lo, hi) (uint64), freed with cpx_free. */
 int cpx_get_device_primes(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t **d_out, size_t *len);
+/* CudaSieve::getDevicePrimes32(bottom, top, len)  (call site src/pitable.cu:13,
+ * src/Deleglise-Rivat/omega12_host.cu:62): the same as uint32, hi <= 2^32. */
+int cpx_get_device_primes32(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint32_t **d_out, size_t *len);
+/* CudaSieve::getHostPrimes(bottom, top, len)  (call site src/S3.cu:27): the
+ * primes of [lo, hi) into the caller's buffer out[cap]; *len = their number.
+ * out == NULL only counts; cap < *len gives CPX_EINVAL (nothing written). */
+int cpx_get_host_primes(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t *out, size_t cap, size_t *len);
+/* CudaSieve::genDeviceBitSieve(bottom, top)  (call sites
+ * src/sieve/S2_hard_host.cu:65, src/Deleglise-Rivat/omega12_host.cu:63):
+ * odd-only bit sieve of [lo, hi), lo a multiple of 64: bit i of word j stands
+ * for n = lo + 64 j + 2 i + 1 and is CLEAR iff n is an odd prime in [lo, hi)
+ * (the reader at src/sieve/S2_hard_device.cu:248-249 takes a clear bit as
+ * prime); bits of 1, composites and n >= hi are set.  *nwords =
+ * ceil((hi - lo) / 64); freed with cpx_free.  Ranges of any size to
+ * 2^64 - 2^34 are sieved in 2^20-integer segments with sieving primes to
+ * sqrt(hi) only (so are cpx_get_device_primes above 2^24, _primes32 and
+ * cpx_get_host_primes, unless the pi table already reaches hi). */
+int cpx_gen_device_bit_sieve(cpx_ctx *ctx, uint64_t lo, uint64_t hi, uint32_t **d_words, size_t *nwords);
 /* uint32_t* get_d_piTable(uint32_t hi)   include/pitable.cuh:7: u32 pi(n) at
  * index (n+1)/2 for n < hi; freed with cpx_free. */
 int cpx_get_d_pitable(cpx_ctx *ctx, uint32_t hi, uint32_t **d_out);

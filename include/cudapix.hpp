@@ -8,6 +8,8 @@
 // device; uint128_t is this library's own 128-bit type (unsigned __int128).
 #pragma once
 #include <cstdint>
+#include <cstdlib>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include "cudapix.h"
@@ -111,6 +113,54 @@ inline uint32_t *get_d_piTable(uint32_t hi)
   cpx::check(cpx_get_d_pitable(cpx::default_ctx(), hi, &d));
   return d;   // free with cpx_free(cpx::default_ctx(), d)
 }
+
+// CUDASieve's calls as the reference makes them (external library; call
+// sites src/P2.cu:100,103, src/pitable.cu:13,147, src/S3.cu:27,
+// src/sieve/S2_hard_host.cu:65, src/V.cu:64-65): ranges are [bottom, top);
+// returned arrays are freed with cpx_free(cpx::default_ctx(), p), host
+// arrays with free().
+namespace CudaSieve {
+inline uint64_t countPrimes(uint64_t bottom, uint64_t top, int device = 0)
+{
+  (void)device;
+  uint64_t n = 0;
+  cpx::check(cpx_count_primes(cpx::default_ctx(), bottom, top, &n));
+  return n;
+}
+inline uint64_t countPrimes(uint64_t top) { return countPrimes(0, top); }
+inline uint64_t *getDevicePrimes(uint64_t bottom, uint64_t top, size_t &len, int device = 0)
+{
+  (void)device;
+  uint64_t *d = nullptr;
+  cpx::check(cpx_get_device_primes(cpx::default_ctx(), bottom, top, &d, &len));
+  return d;
+}
+inline uint32_t *getDevicePrimes32(uint64_t bottom, uint64_t top, size_t &len, int device = 0)
+{
+  (void)device;
+  uint32_t *d = nullptr;
+  cpx::check(cpx_get_device_primes32(cpx::default_ctx(), bottom, top, &d, &len));
+  return d;
+}
+inline uint64_t *getHostPrimes(uint64_t bottom, uint64_t top, size_t &len, int device = 0)
+{
+  (void)device;
+  cpx::check(cpx_get_host_primes(cpx::default_ctx(), bottom, top, nullptr, 0, &len));
+  uint64_t *h = (uint64_t *)malloc((len ? len : 1) * sizeof(uint64_t));
+  if (!h) throw std::bad_alloc();
+  size_t n = 0;
+  cpx::check(cpx_get_host_primes(cpx::default_ctx(), bottom, top, h, len, &n));
+  return h;
+}
+inline uint32_t *genDeviceBitSieve(uint64_t bottom, uint64_t top, int device = 0)
+{
+  (void)device;
+  uint32_t *d = nullptr;
+  size_t nw = 0;
+  cpx::check(cpx_gen_device_bit_sieve(cpx::default_ctx(), bottom, top, &d, &nw));
+  return d;
+}
+}  // namespace CudaSieve
 
 // include/pitable.cuh:12-51
 class PiTable {

@@ -25,8 +25,24 @@ uint64_t dr64(uint64_t x, uint64_t y, uint16_t c) {
 uint128_t dr128(uint128_t x, uint64_t y, uint16_t c) {
   return Ordinary(x, y, c) + S2_trivial(x, y) - P2(x, y);
 }
+// CUDASieve as the reference calls it (src/P2.cu:100, src/pitable.cu:13, src/S3.cu:27,
+// src/sieve/S2_hard_host.cu:65, src/V.cu:64)
+uint64_t sieve_calls(uint64_t y) {
+  size_t len = 0, len32 = 0, hl = 0;
+  uint64_t *d = CudaSieve::getDevicePrimes(0, y, len, 0);
+  uint32_t *d32 = CudaSieve::getDevicePrimes32(0, y, len32, 0);
+  uint64_t *h = CudaSieve::getHostPrimes(0ull, y, hl, 0);
+  uint32_t *bits = CudaSieve::genDeviceBitSieve(0, y, 0);
+  uint64_t n = CudaSieve::countPrimes(0, y, 0) + CudaSieve::countPrimes(y);
+  cpx_free(cpx::default_ctx(), d);
+  cpx_free(cpx::default_ctx(), d32);
+  cpx_free(cpx::default_ctx(), bits);
+  free(h);
+  return n + len + len32 + hl;
+}
 int main(int argc, char **argv) {
   if (argc > 1) {                       // only with a device
+    std::printf("%llu\n", (unsigned long long)sieve_calls(100000));
     PiTable t(1000000, 0, 1000000);
     Phi phi(100000);
     std::printf("%llu %llu\n", (unsigned long long)dr64(1000000000000ull, 20000, 6),

@@ -39,6 +39,97 @@ def test_get_device_primes(ctx, oracle):
     assert got == ref
 
 
+def _sieve_range(lo, hi, oracle):
+    """Primes of [lo, hi) by numpy Eratosthenes with the oracle's primes to sqrt(hi)."""
+    import math
+    import numpy as np
+    a = np.ones(hi - lo, dtype=bool)
+    for p in oracle.primes(math.isqrt(hi) + 1):
+        s0 = max(p * p, (lo + p - 1) // p * p)
+        a[s0 - lo::p] = False
+    return [lo + int(i) for i in np.flatnonzero(a) if lo + int(i) >= 2]
+
+
+def _is_prime_u64(n):
+    """Deterministic Miller-Rabin for n < 2^64 (bases 2, 325, 9375, 28178,
+    450775, 9780504, 1795265022)."""
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for a in (2, 325, 9375, 28178, 450775, 9780504, 1795265022):
+        a %= n
+        if a == 0:
+            continue
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_range_primes_segmented(oracle):
+    # ranges above 2^24 the pi table does not reach go through the segmented
+    # bit sieve (2^20 integers per block, sieving primes to sqrt(hi) only):
+    # across segment and word boundaries, unaligned ends, one prime, none
+    # (a fresh context, whose pi table reaches none of them)
+    import cudapix_amd as C
+    with C.Context(device=0) as ctx:
+        _range_primes(ctx, oracle)
+
+
+def _range_primes(ctx, oracle):
+    cases = [(2 ** 24 - 1000, 2 ** 24 + 5000), (10 ** 12, 10 ** 12 + 3 * 2 ** 20 + 7),
+             (10 ** 12 + 37, 10 ** 12 + 37 + 2 ** 20 + 13), (10 ** 13 + 1, 10 ** 13 + 2),
+             (10 ** 13 + 37, 10 ** 13 + 38), (3 * 10 ** 13 + 1, 3 * 10 ** 13 + 64)]
+    for lo, hi in cases:
+        print("case getDevicePrimes[%d,%d)" % (lo, hi), flush=True)
+        want = _sieve_range(lo, hi, oracle)
+        assert ctx.getDevicePrimes(lo, hi) == want, (lo, hi)
+        assert ctx.countPrimes(lo, hi) == len(want)
+    assert ctx.getHostPrimes(10 ** 12, 10 ** 12 + 100000) == _sieve_range(10 ** 12, 10 ** 12 + 100000, oracle)
+    assert ctx.getHostPrimes(0, 100) == oracle.primes(99)
+    # near 2^63: every odd n of the range classified by Miller-Rabin
+    lo, hi = 2 ** 63 - 30001, 2 ** 63 + 10000
+    print("case getDevicePrimes near 2^63", flush=True)
+    got = ctx.getDevicePrimes(lo, hi)
+    assert got == [n for n in range(lo, hi) if _is_prime_u64(n)]
+
+
+def test_device_primes32_and_bit_sieve(ctx, oracle):
+    import cudapix_amd as C
+    assert ctx.getDevicePrimes32(0, 10 ** 6) == oracle.primes(10 ** 6 - 1)
+    assert ctx.getDevicePrimes32(2 ** 32 - 100000, 2 ** 32) == _sieve_range(2 ** 32 - 100000, 2 ** 32, oracle)
+    with pytest.raises(C.CpxError) as e:
+        ctx.getDevicePrimes32(0, 2 ** 32 + 1)
+    assert e.value.code == -4
+    # genDeviceBitSieve(0, y) as the hard-leaf code reads it: bit (m % 64) / 2
+    # of word m / 64 is clear iff odd m < y is prime
+    for lo, hi in ((0, 10 ** 6 + 7), (2 ** 40, 2 ** 40 + 3 * 2 ** 20 + 100)):
+        print("case genDeviceBitSieve[%d,%d)" % (lo, hi), flush=True)
+        words = ctx.genDeviceBitSieve(lo, hi)
+        assert len(words) == (hi - lo + 63) // 64
+        pr = set(_sieve_range(lo, hi, oracle))
+        for w, word in enumerate(words):
+            for i in range(32):
+                n = lo + 64 * w + 2 * i + 1
+                want = 0 if (n < hi and n in pr) else 1
+                assert (word >> i) & 1 == want, n
+    with pytest.raises(C.CpxError) as e:
+        ctx.genDeviceBitSieve(10, 1000)
+    assert e.value.code == -1
+
+
 def test_get_d_pitable_layout(ctx, oracle):
     hi = 200000
     tab = ctx.get_d_piTable(hi)
