@@ -128,6 +128,11 @@ struct HardRowInfo {
   uint64_t moff;
 };
 
+// Hard-leaf stages per carry block (k_hard_fin): a row's T and sum(mu) of
+// consecutive stages are adjacent, so a block that takes several stages
+// reads them from the same cache lines.
+#define CPX_FIN_STAGES 8
+
 // Block descriptor of the grid-stride term kernels (ordinary, trivial, easy).
 struct BlockDesc {
   uint32_t query;

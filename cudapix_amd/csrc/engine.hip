@@ -318,6 +318,7 @@ static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint
 struct QPlan {
   QueryDev d{};
   std::vector<BlockDesc> ord, triv, easy, fin, p2d;
+  std::vector<uint32_t> fin_rows;    // rows each carry block scans (its first stage's active rows)
   std::vector<HardRowInfo> rowinfo;  // replicas and M rows are set per group
   std::vector<float> rleaves;        // estimated leaves of each hard row
   std::vector<float> rleafy;         // estimated leafy stages of each hard row
@@ -342,6 +343,7 @@ struct Plan {
   std::vector<GroupDev> groups;
   std::vector<QPlan> qps;
   std::vector<PlanOff> off;
+  std::vector<BlockDesc> fin;        // carry blocks of all queries, longest first
   size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, np2d = 0, nrows = 0;
   uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
   uint64_t hard_mrows = 0, hard_mm = 0;
@@ -435,8 +437,9 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
     }
   }
   d.n_easy_parts = part - d.n_ord_parts - d.n_triv_parts;
-  // hard carry: one block per stage
-  for (uint64_t b = 0; b < d.nstage; b++) qp.fin.push_back({i, part++, b, b + 1});
+  // hard carry: one block per CPX_FIN_STAGES stages
+  for (uint64_t b = 0; b < d.nstage; b += CPX_FIN_STAGES)
+    qp.fin.push_back({i, part++, b, std::min<uint64_t>(d.nstage, b + CPX_FIN_STAGES)});
   d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
   // P2 by lookups: prime indices (pi(y), pi(sqrt x)] (primes[k] is the (k+1)-th prime)
   if (d.p2tab) {
@@ -493,6 +496,13 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
     toff += act;
   }
   qp.tm = toff;
+  // carry blocks: rows with their first stage active (act is non-increasing)
+  qp.fin_rows.resize(qp.fin.size());
+  uint64_t r = d.hard_nseg;
+  for (size_t k = 0; k < qp.fin.size(); k++) {
+    while (r > 0 && qp.rowinfo[r - 1].act <= qp.fin[k].begin) r--;
+    qp.fin_rows[k] = (uint32_t)r;
+  }
 }
 
 // Groups of queries that share the segment sieve: queries with hard rows,
@@ -741,6 +751,16 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   pl.tiles.resize(ntiles);
   for (const GPlan &gp : gps)
     for (size_t k = 0; k < gp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - gp.tbucket[k]]++] = gp.tiles[k];
+  // carry blocks longest first too (a block's time is its chain of row
+  // chunks), so the grid-stride loop does not end on a few long ones
+  std::fill(start.begin(), start.end(), 0u);
+  for (const QPlan &qp : pl.qps)
+    for (uint32_t r : qp.fin_rows) start[PLAN_NB - 1 - cost_bucket(1.0 + r) + 1]++;
+  for (int b = 0; b < PLAN_NB; b++) start[b + 1] += start[b];
+  pl.fin.resize(pl.nfin);
+  for (const QPlan &qp : pl.qps)
+    for (size_t k = 0; k < qp.fin.size(); k++)
+      pl.fin[start[PLAN_NB - 1 - cost_bucket(1.0 + qp.fin_rows[k])]++] = qp.fin[k];
   return CPX_OK;
 }
 
@@ -903,6 +923,7 @@ static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
   put(L.queries, pl.q.data(), pl.q.size() * sizeof(QueryDev));
   put(L.tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDev));
   put(L.groups, pl.groups.data(), pl.groups.size() * sizeof(GroupDev));
+  put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
   // every query's descriptors and rows straight to their places
   parallel_for(pl.qps.size(), plan_threads(pl.qps.size()), [&](size_t i) {
     const QPlan &qp = pl.qps[i];
@@ -911,7 +932,6 @@ static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
     put(L.triv + o.triv * sizeof(BlockDesc), qp.triv.data(), qp.triv.size() * sizeof(BlockDesc));
     put(L.easy + o.easy * sizeof(BlockDesc), qp.easy.data(), qp.easy.size() * sizeof(BlockDesc));
     put(L.p2d + o.p2d * sizeof(BlockDesc), qp.p2d.data(), qp.p2d.size() * sizeof(BlockDesc));
-    put(L.fin + o.fin * sizeof(BlockDesc), qp.fin.data(), qp.fin.size() * sizeof(BlockDesc));
     put(L.rowinfo + o.rows * sizeof(HardRowInfo), qp.rowinfo.data(), qp.rowinfo.size() * sizeof(HardRowInfo));
   });
   // copy only the used prefix of each array

@@ -250,51 +250,94 @@ hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s)
 }
 
 // -------------------------------------------------------- hard carry
-// Hard-leaf carry of one stage s of one query per block:
+// Hard-leaf carry of stages [s0, s0 + ns) (ns <= CPX_FIN_STAGES) of one
+// query per block: for each of them
 //   carry_s = sum over segments r (in order, while s is active in r) of
 //             msum_s(r) * (T_s(0) + ... + T_s(r-1))
 // with msum_s(r) the replicas' sum(mu) of the leaves of p_s in segment r and
 // T_s(r) its survivors before stage s.  Activity is a prefix of the segments
 // (bact is non-increasing in r), so the block scans 256 segments at a time
-// and stops after the first chunk that ends inactive.
+// and stops after the first chunk that ends inactive for s0.  A thread reads
+// its segment's ns consecutive T and sum(mu) entries, which share cache lines
+// (the row record is read once for all of them).
 __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
 {
+  constexpr uint32_t FS = CPX_FIN_STAGES;
   __shared__ i128 red[4];
-  __shared__ unsigned long long wt[4];
+  __shared__ unsigned long long wt[FS][4];
   __shared__ uint32_t stop;
   const uint32_t nd = *bd.n_fin;
   const uint32_t t = threadIdx.x;
   for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
     const BlockDesc d = bd.fin_desc[di];
     const QueryDev &q = bd.queries[d.query];
-    const uint64_t s = d.begin;
+    const uint64_t s0 = d.begin;
+    const uint32_t ns = (uint32_t)(d.end - d.begin);           // 1..FS, block-uniform
     i128 carry = 0;
-    unsigned long long base = 0;
-    for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += 256) {
-      const uint64_t r = r0 + t;
-      unsigned long long tv = 0;
-      long long msum = 0;
-      bool act = false;
+    unsigned long long base[FS];
+#pragma unroll
+    for (uint32_t j = 0; j < FS; j++) base[j] = 0;
+    // the next chunk's row records are fetched while this chunk is summed
+    uint32_t ba_n = 0;
+    HardRowInfo ri_n{};
+    auto fetch = [&](uint64_t r) {
+      ba_n = 0;
       if (r < q.hard_nseg) {
-        const uint64_t row = q.hard_row0 + r;
-        if (bd.hard_bact[row] > s) {
+        ba_n = bd.hard_bact[q.hard_row0 + r];
+        if (ba_n > s0) ri_n = bd.rowinfo[q.hard_row0 + r];
+      }
+    };
+    fetch(t);
+    for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += 256) {
+      const uint32_t ba = ba_n;
+      const HardRowInfo ri = ri_n;
+      fetch(r0 + 256 + t);
+      unsigned long long tv[FS], incl[FS];
+      long long msum[FS];
+#pragma unroll
+      for (uint32_t j = 0; j < FS; j++) { tv[j] = 0; msum[j] = 0; }
+      bool act = false;
+      {
+        if (ba > s0) {
           act = true;
-          const HardRowInfo ri = bd.rowinfo[row];
-          tv = bd.hard_T[q.tm_off + ri.toff + s];
-          const int32_t *m = bd.hard_M + q.m_off + ri.moff + s;
-          for (uint32_t k = 0; k < ri.nrep; k++) msum += m[(uint64_t)k * ri.act];
+          const uint32_t *T = bd.hard_T + q.tm_off + ri.toff + s0;
+          const int32_t *m = bd.hard_M + q.m_off + ri.moff + s0;
+          const uint32_t nj = min(ns, ba - (uint32_t)s0);   // this row's stages of the block
+#pragma unroll
+          for (uint32_t j = 0; j < FS; j++)
+            if (j < nj) tv[j] = T[j];
+          // replicas outermost: each step's nj loads are independent and
+          // adjacent, so they are in flight together
+#pragma unroll 2
+          for (uint32_t k = 0; k < ri.nrep; k++) {
+            const int32_t *mk = m + (uint64_t)k * ri.act;
+#pragma unroll
+            for (uint32_t j = 0; j < FS; j++)
+              if (j < nj) msum[j] += mk[j];
+          }
         }
       }
-      const unsigned long long incl = wave_incl_scan(tv);
-      if (lane_id() == CPX_WAVE - 1) wt[wave_id()] = incl;
-      __syncthreads();
-      unsigned long long pre = incl - tv, tot = 0;
-      for (uint32_t w = 0; w < 4; w++) {
-        tot += wt[w];
-        if (w < wave_id()) pre += wt[w];
+#pragma unroll
+      for (uint32_t j = 0; j < FS; j++) {
+        incl[j] = 0;
+        if (j < ns) {
+          incl[j] = wave_incl_scan(tv[j]);
+          if (lane_id() == CPX_WAVE - 1) wt[j][wave_id()] = incl[j];
+        }
       }
-      carry += (i128)(base + pre) * msum;
-      base += tot;
+      __syncthreads();
+#pragma unroll
+      for (uint32_t j = 0; j < FS; j++) {
+        if (j < ns) {
+          unsigned long long pre = incl[j] - tv[j], tot = 0;
+          for (uint32_t w = 0; w < 4; w++) {
+            tot += wt[j][w];
+            if (w < wave_id()) pre += wt[j][w];
+          }
+          carry += (i128)(base[j] + pre) * msum[j];
+          base[j] += tot;
+        }
+      }
       if (t == 255) stop = act ? 0u : 1u;
       __syncthreads();
       if (stop) break;
