@@ -87,6 +87,15 @@ def _pipelined(C, ctx, oracle):
     batches = [b2, b1[:3], b3, b1, b2[:1]]
     assert list(ctx.pi_batches(batches)) == [ctx.pi_batch(b) for b in batches]
     assert list(ctx.pi_batches(batches, use_graph=False)) == [ctx.pi_batch(b) for b in batches]
+    # a submission that grows the resident tables while the other slot's
+    # batch is still queued (the old tables are freed only after it ran)
+    big = [10 ** 16, 10 ** 16 + 1, 3 * 10 ** 15]
+    ta = ctx.pi_batch_submit(b1)
+    tb = ctx.pi_batch_submit(big)
+    assert ctx.pi_batch_collect(ta) == want[0]
+    got = ctx.pi_batch_collect(tb)
+    assert got[0] == 279238341033925
+    assert got == [ctx.pi(x) for x in big]
 
 
 def test_p2_lookup_equals_p2_segments():
