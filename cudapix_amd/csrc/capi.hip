@@ -831,7 +831,7 @@ int cpx_autotune(cpx_ctx *c, const cpx_u128 *xs, size_t n, uint32_t *threads, ui
   if (!xs || !n) return CPX_EINVAL;
   static const SegShape cands[] = {{64, 32}, {64, 64}, {64, 128}, {128, 32}, {128, 64}, {256, 32},
                                    {256, 64}, {256, 128}, {512, 32}, {512, 64}};
-  static const double scales[] = {0.25, 0.3, 0.4, 0.5, 0.6, 0.7, 0.85, 1.0, 1.2, 1.45};
+  static const double scales[] = {0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6, 0.7, 0.85, 1.0, 1.2, 1.45};
   static const uint32_t wheels[] = {6, 7, 8};
   std::vector<HostQuery> hq(n);
   auto make = [&]() -> int {

@@ -109,11 +109,15 @@ struct GroupDev {
 // segment but evaluates only the leaves with index = rep (mod nrep), writing
 // its own sum(mu) rows.
 struct TileDev {
-  uint32_t group;
+  uint32_t group;        // CPX_TILE_EASY: easy-leaf tile of descriptor `seg`
   uint16_t nrep;
   uint16_t rep;
   uint64_t seg;
 };
+
+// TileDev.group of an easy-leaf (V) tile: the persistent kernel interleaves
+// them with the segment tiles, so their gathers overlap the sieve's ALU work.
+#define CPX_TILE_EASY 0xffffffffu
 
 // Per hard segment (query-relative): its first M-row, number of replicas,
 // number of active stages `act` (p^2 <= x/lo) and where its counters live:
@@ -192,7 +196,6 @@ hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s);
 hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sprimes,
                                  uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
                                  hipStream_t s);
-int easy_blocks_per_cu();
 hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_recip(const uint32_t *primes, double *rinv, uint64_t n, hipStream_t s);
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes, uint64_t cap,
@@ -216,7 +219,6 @@ hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t launch_setup(BatchDev b, uint32_t maxq, uint32_t max_stage_total, hipStream_t s);
 hipError_t launch_ordinary(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s);
-hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_hard_finalize(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_combine(BatchDev b, uint32_t maxq, hipStream_t s);
 

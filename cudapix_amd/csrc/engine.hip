@@ -290,7 +290,7 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
-static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 16384, P2_CH = 8192;
+static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 65536, P2_CH = 8192;
 
 static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint64_t SB,
                        uint64_t *R, uint64_t *P, uint64_t *Y)
@@ -751,6 +751,19 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   pl.tiles.resize(ntiles);
   for (const GPlan &gp : gps)
     for (size_t k = 0; k < gp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - gp.tbucket[k]]++] = gp.tiles[k];
+  // easy-leaf tiles (one per easy descriptor, global index in `seg`) spread
+  // evenly through the segment tiles, so that CUs run both kinds at once
+  if (pl.neasy) {
+    std::vector<TileDev> merged;
+    merged.reserve(pl.tiles.size() + pl.neasy);
+    const uint64_t ns = pl.tiles.size(), ne = pl.neasy;
+    uint64_t is = 0, ie = 0;
+    while (is < ns || ie < ne) {
+      if (ie < ne && (is == ns || ie * ns <= is * ne)) merged.push_back({CPX_TILE_EASY, 1, 0, ie++});
+      else merged.push_back(pl.tiles[is++]);
+    }
+    pl.tiles.swap(merged);
+  }
   // carry blocks longest first too (a block's time is its chain of row
   // chunks), so the grid-stride loop does not end on a few long ones
   std::fill(start.begin(), start.end(), 0u);
@@ -970,19 +983,10 @@ static int seg_grid(cpx_ctx *c)
 
 static const uint32_t GRID_Q = 256, GRID_T = 2048;
 
-// The easy-leaf kernel's grid: one resident round of blocks (a second,
-// partial round would leave most of the chip idle at its end).
-static uint32_t easy_grid(cpx_ctx *c)
-{
-  static int nb = -1;
-  if (nb < 0) nb = easy_blocks_per_cu();
-  return nb > 0 ? (uint32_t)(nb * c->cu_count) : GRID_T;
-}
-
 // The launch sequence in three parts; events between the parts time the
 // persistent segment kernel on its own stream (also when replayed as graphs).
-//   part 0: clear results + queue head, setup, ordinary, trivial, easy
-//   part 1: persistent segment kernel (hard + P2 tiles)
+//   part 0: clear results + queue head, setup, ordinary, trivial, P2 lookups
+//   part 1: persistent segment kernel (hard + P2 tiles, easy-leaf tiles)
 //   part 2: hard-leaf carry, combine
 static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_words)
 {
@@ -992,7 +996,6 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
     HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
     HIPCHK(launch_ordinary(b, GRID_T, c->stream));
     HIPCHK(launch_trivial(b, GRID_T, c->stream));
-    HIPCHK(launch_easy(b, easy_grid(c), c->stream));
     HIPCHK(launch_p2tab(b, GRID_T, c->stream));
   } else if (part == 1) {
     HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
@@ -1012,7 +1015,6 @@ static int ensure_graph(cpx_ctx *c, BatchSlot &S, const BatchDev &b, uint32_t nr
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   // kernel attributes and the occupancy queries must happen outside capture
   (void)seg_grid(c);
-  (void)easy_grid(c);
   for (int part = 0; part < 3; part++) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

@@ -887,6 +887,92 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
   if (rep == 0 && p2any) p2_epilogue<T, K>(bd, grp, seg, lo, c + nst, lds, sh);
 }
 
+// Easy leaves (replaces S2_a / V_a / V_b, src/Deleglise-Rivat/S2.cu:72-99,
+// src/V.cu:22-124) of one descriptor, as a tile of the persistent kernel:
+//   V = sum_{x^(1/4) < p <= cbrt x} sum_{p < q <= min(y, x/p^2)} [pi(x/pq) - pi(p) + 2]
+// over the descriptor's primes p = primes[j], j in [begin, end) (all these
+// leaves have x/pq < p^2).  The block sets up T primes at a time in parallel
+// (x/p, the q range, the (2 - pi(p)) terms), then strides its threads over
+// each prime's q's with U independent prime -> quotient -> pi-table chains
+// per thread (U * T = 1024 slots when T <= 256).  Uses the first 3T words of
+// the segment LDS.
+template <int T, int K>
+__device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint32_t *lds, SegShared<T, K> &sh)
+{
+  constexpr uint32_t U = T >= 512 ? 2 : 4;
+  uint64_t *sxp = reinterpret_cast<uint64_t *>(lds);    // x / p of the chunk's primes
+  uint32_t *sk1 = lds + 2 * T;                          // pi(min(y, x / p^2)): q indices end here
+  const uint32_t t = threadIdx.x;
+  const BlockDesc d = bd.easy_desc[di];
+  const QueryDev &q = bd.queries[d.query];
+  const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+  i128 sum = 0;
+  for (uint64_t j0 = d.begin; j0 < d.end; j0 += T) {
+    const uint32_t nj = (uint32_t)min((uint64_t)T, d.end - j0);
+    if (t < nj) {
+      const uint64_t j = j0 + t;
+      const uint64_t p = bd.primes[j];
+      const uint64_t xp = cpx_div128_to64(x, p);
+      uint64_t qmax = q.y;
+      if ((xp >> 50) < p) {
+        const uint64_t v = cpx_div_smallq(xp, p);
+        if (v < qmax) qmax = v;
+      }
+      const uint64_t k1 = pi_lookup(bd.pitab, qmax);
+      sxp[t] = xp;
+      sk1[t] = (uint32_t)k1;
+      if (k1 > j + 1) sum += (i128)(k1 - j - 1) * (1 - (i128)j);   // (2 - pi(p)) per q, pi(p) = j + 1
+    }
+    __syncthreads();
+    for (uint32_t jj = 0; jj < nj; jj++) {
+      const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
+      if (k1 <= j + 1) continue;
+      // a chain past k1 repeats index k and is masked; x/(p q) < x/p^2 <
+      // 2^50, so the reciprocal divide with the tabulated 1/q is exact
+      const double xpf = (double)xp;
+      uint64_t s = 0;
+      for (uint64_t k = j + 1 + t; k < k1; k += U * T) {
+        uint32_t qv[U];
+        double rv[U];
+        bool ok[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+          ok[u] = k + u * T < k1;
+          const uint64_t kk = ok[u] ? k + u * T : k;
+          qv[u] = bd.primes[kk];
+          rv[u] = bd.prinv[kk];
+        }
+        PiEntry pe[U];
+        uint32_t ix[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+          const uint64_t v = cpx_div_leaf_r(xp, xpf, qv[u], rv[u]);
+          ix[u] = (uint32_t)(((v & 63) + 1) >> 1) | (v >= 2 ? 0x80u : 0u);
+          pe[u] = bd.pitab[v >> 6];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+          const uint32_t c = (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
+          s += ok[u] ? c : 0u;
+        }
+      }
+      sum += (i128)s;
+    }
+    __syncthreads();
+  }
+  // block sum: wave sums, then wave 0 adds them
+  sum = wave_sum_i128(sum);
+  i128 *red = reinterpret_cast<i128 *>(lds);
+  if (lane_id() == 0) red[wave_id()] = sum;
+  __syncthreads();
+  if (t == 0) {
+    i128 tot = 0;
+    for (uint32_t w = 0; w < T / CPX_WAVE; w++) tot += red[w];
+    bd.parts[q.part_off + d.part] = tot;
+  }
+  (void)sh;
+}
+
 template <int T, int K>
 // (at least 4 waves per SIMD: two 512-thread blocks per CU need <= 128 VGPRs)
 __global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
@@ -900,7 +986,8 @@ __global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
     const uint32_t ti = sh.tile;
     if (ti >= ntiles) break;
     const TileDev tile = bd.tiles[ti];
-    hard_segment<T, K>(bd, tile, lds, sh);
+    if (tile.group == CPX_TILE_EASY) easy_tile<T, K>(bd, (uint32_t)tile.seg, lds, sh);
+    else hard_segment<T, K>(bd, tile, lds, sh);
     __syncthreads();
   }
 }

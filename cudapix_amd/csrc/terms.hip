@@ -124,99 +124,6 @@ hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s)
   return hipGetLastError();
 }
 
-// --------------------------------------------------------------- easy V
-// V = sum_{x^(1/4) < p <= cbrt x} sum_{p < q <= min(y, x/p^2)} [pi(x/pq) - pi(p) + 2]
-// (src/V.cu:22-124; all these leaves have x/pq < p^2).  One descriptor per
-// group of primes (prime indices [begin, end), grouped by q-count on the
-// host); the block sets up 256 primes at a time in parallel (x/p, the q
-// range) and then strides its threads over each prime's q's.
-__global__ __launch_bounds__(256) void k_easy(BatchDev bd)
-{
-  __shared__ i128 red[4];
-  __shared__ uint64_t sxp[256];          // x / p of the chunk's primes
-  __shared__ uint32_t sk1[256];          // pi(min(y, x / p^2)): q indices end here
-  const uint32_t nd = *bd.n_easy;
-  const uint32_t t = threadIdx.x;
-  const uint64_t B = blockDim.x;
-  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
-    const BlockDesc d = bd.easy_desc[di];
-    const QueryDev &q = bd.queries[d.query];
-    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
-    i128 sum = 0;
-    for (uint64_t j0 = d.begin; j0 < d.end; j0 += 256) {
-      const uint32_t nj = (uint32_t)min((uint64_t)256, d.end - j0);
-      // per prime, in parallel: x/p, the q range, and the (2 - pi(p)) terms
-      if (t < nj) {
-        const uint64_t j = j0 + t;
-        const uint64_t p = bd.primes[j];
-        const uint64_t xp = cpx_div128_to64(x, p);
-        uint64_t qmax = q.y;
-        if ((xp >> 50) < p) {
-          uint64_t v = cpx_div_smallq(xp, p);
-          if (v < qmax) qmax = v;
-        }
-        const uint64_t k1 = pi_lookup(bd.pitab, qmax);
-        sxp[t] = xp;
-        sk1[t] = (uint32_t)k1;
-        if (k1 > j + 1) sum += (i128)(k1 - j - 1) * (1 - (i128)j);   // (2 - pi(p)) per q, pi(p) = j + 1
-      }
-      __syncthreads();
-      for (uint32_t jj = 0; jj < nj; jj++) {
-        const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
-        if (k1 <= j + 1) continue;
-        // four independent prime -> quotient -> pi-table chains per thread
-        // in flight, without branches (a chain past k1 repeats index k and
-        // is masked); x/(p q) < x/p^2 < 2^50, so the reciprocal divide with
-        // the tabulated 1/q is exact
-        const double xpf = (double)xp;
-        uint64_t s = 0;
-        for (uint64_t k = j + 1 + t; k < k1; k += 4 * B) {
-          uint32_t qv[4];
-          double rv[4];
-          bool ok[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            ok[u] = k + u * B < k1;
-            const uint64_t kk = ok[u] ? k + u * B : k;
-            qv[u] = bd.primes[kk];
-            rv[u] = bd.prinv[kk];
-          }
-          PiEntry pe[4];
-          uint32_t ix[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const uint64_t v = cpx_div_leaf_r(xp, xpf, qv[u], rv[u]);
-            ix[u] = (uint32_t)(((v & 63) + 1) >> 1) | (v >= 2 ? 0x80u : 0u);
-            pe[u] = bd.pitab[v >> 6];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const uint32_t c = (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
-            s += ok[u] ? c : 0u;
-          }
-        }
-        sum += (i128)s;
-      }
-      __syncthreads();
-    }
-    i128 tt = block_sum_i128(sum, red);
-    if (t == 0) bd.parts[q.part_off + d.part] = tt;
-  }
-}
-
-int easy_blocks_per_cu()
-{
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_easy, 256, 0) != hipSuccess) nb = 0;
-  return nb;
-}
-
-hipError_t launch_easy(BatchDev b, uint32_t grid, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_easy, dim3(grid), dim3(256), 0, s, b);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------- P2 by lookups
 // Sum of pi(x/p) over a range of P2 primes p = primes[k] (y < p <= sqrt x,
 // x < 2^64), one block per descriptor: x/p by the reciprocal divide with the
