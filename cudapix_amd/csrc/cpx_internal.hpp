@@ -109,15 +109,17 @@ struct GroupDev {
 // segment but evaluates only the leaves with index = rep (mod nrep), writing
 // its own sum(mu) rows.
 struct TileDev {
-  uint32_t group;        // CPX_TILE_EASY: easy-leaf tile of descriptor `seg`
+  uint32_t group;        // CPX_TILE_EASY / CPX_TILE_P2: term tile of descriptor `seg`
   uint16_t nrep;
   uint16_t rep;
   uint64_t seg;
 };
 
-// TileDev.group of an easy-leaf (V) tile: the persistent kernel interleaves
-// them with the segment tiles, so their gathers overlap the sieve's ALU work.
+// TileDev.group of an easy-leaf (V) tile and of a P2-lookup tile (`seg` is
+// the descriptor index): the persistent kernel interleaves them with the
+// segment tiles, so their gathers overlap the sieve's ALU work.
 #define CPX_TILE_EASY 0xffffffffu
+#define CPX_TILE_P2 0xfffffffeu
 
 // Per hard segment (query-relative): its first M-row, number of replicas,
 // number of active stages `act` (p^2 <= x/lo) and where its counters live:
@@ -196,7 +198,6 @@ hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s);
 hipError_t launch_pitab_segments(PiEntry *tab, uint64_t nent, const uint32_t *sprimes,
                                  uint32_t nsprimes, uint32_t *seg_tot, uint64_t *seg_base,
                                  hipStream_t s);
-hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_recip(const uint32_t *primes, double *rinv, uint64_t n, hipStream_t s);
 hipError_t launch_extract_primes(const PiEntry *tab, uint64_t limit, uint32_t *primes, uint64_t cap,
                                  hipStream_t s);

@@ -290,7 +290,7 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
-static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 65536, P2_CH = 8192;
+static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 65536, P2_CH = 65536;
 
 static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint64_t SB,
                        uint64_t *R, uint64_t *P, uint64_t *Y)
@@ -751,19 +751,23 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   pl.tiles.resize(ntiles);
   for (const GPlan &gp : gps)
     for (size_t k = 0; k < gp.tiles.size(); k++) pl.tiles[start[PLAN_NB - 1 - gp.tbucket[k]]++] = gp.tiles[k];
-  // easy-leaf tiles (one per easy descriptor, global index in `seg`) spread
-  // evenly through the segment tiles, so that CUs run both kinds at once
-  if (pl.neasy) {
+  // easy-leaf and P2-lookup tiles (one per descriptor, its global index in
+  // `seg`) spread evenly through the segment tiles, so that CUs run gathers
+  // beside sieve work
+  auto spread = [&](uint32_t kind, uint64_t ne) {
+    if (!ne) return;
     std::vector<TileDev> merged;
-    merged.reserve(pl.tiles.size() + pl.neasy);
-    const uint64_t ns = pl.tiles.size(), ne = pl.neasy;
+    merged.reserve(pl.tiles.size() + ne);
+    const uint64_t ns = pl.tiles.size();
     uint64_t is = 0, ie = 0;
     while (is < ns || ie < ne) {
-      if (ie < ne && (is == ns || ie * ns <= is * ne)) merged.push_back({CPX_TILE_EASY, 1, 0, ie++});
+      if (ie < ne && (is == ns || ie * ns <= is * ne)) merged.push_back({kind, 1, 0, ie++});
       else merged.push_back(pl.tiles[is++]);
     }
     pl.tiles.swap(merged);
-  }
+  };
+  spread(CPX_TILE_EASY, pl.neasy);
+  spread(CPX_TILE_P2, pl.np2d);
   // carry blocks longest first too (a block's time is its chain of row
   // chunks), so the grid-stride loop does not end on a few long ones
   std::fill(start.begin(), start.end(), 0u);
@@ -985,8 +989,9 @@ static const uint32_t GRID_Q = 256, GRID_T = 2048;
 
 // The launch sequence in three parts; events between the parts time the
 // persistent segment kernel on its own stream (also when replayed as graphs).
-//   part 0: clear results + queue head, setup, ordinary, trivial, P2 lookups
-//   part 1: persistent segment kernel (hard + P2 tiles, easy-leaf tiles)
+//   part 0: clear results + queue head, setup, ordinary, trivial
+//   part 1: persistent segment kernel (hard + P2 segment tiles, easy-leaf
+//           and P2-lookup tiles)
 //   part 2: hard-leaf carry, combine
 static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_words)
 {
@@ -996,7 +1001,6 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
     HIPCHK(launch_setup(b, GRID_Q, 0, c->stream));
     HIPCHK(launch_ordinary(b, GRID_T, c->stream));
     HIPCHK(launch_trivial(b, GRID_T, c->stream));
-    HIPCHK(launch_p2tab(b, GRID_T, c->stream));
   } else if (part == 1) {
     HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
   } else {

@@ -973,6 +973,47 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
   (void)sh;
 }
 
+// P2 by lookups (replaces the windowed upper_bound count of src/P2.cu:76-133)
+// for one descriptor, as a tile of the persistent kernel: the sum of
+// pi(x/p) over P2 primes p = primes[k], k in [begin, end) (y < p <= sqrt x,
+// x < 2^64), x/p by the reciprocal divide with the tabulated 1/p (x/p <=
+// x/(y+1) < 2^50), pi from the resident table, which reaches x/(y+1); four
+// independent chains per thread.  k_combine subtracts sum (pi(p) - 1) in
+// closed form.  Uses the first T/16 words of the segment LDS.
+template <int T, int K>
+__device__ __forceinline__ void p2tab_tile(const BatchDev &bd, uint32_t di, uint32_t *lds)
+{
+  constexpr uint32_t U = 4;
+  const uint32_t t = threadIdx.x;
+  const BlockDesc d = bd.p2_desc[di];
+  const QueryDev &q = bd.queries[d.query];
+  const uint64_t x = q.x_lo;
+  const double xf = (double)x;
+  uint64_t s = 0;
+  for (uint64_t k = d.begin + t; k < d.end; k += U * T) {
+    uint64_t v[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint64_t kk = k + u * T < d.end ? k + u * T : k;
+      v[u] = cpx_div_leaf_r(x, xf, bd.primes[kk], bd.prinv[kk]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint64_t c = pi_lookup(bd.pitab, v[u]);
+      s += k + u * T < d.end ? c : 0u;
+    }
+  }
+  const uint64_t ws = wave_sum(s);
+  uint64_t *red = reinterpret_cast<uint64_t *>(lds);
+  if (lane_id() == 0) red[wave_id()] = ws;
+  __syncthreads();
+  if (t == 0) {
+    i128 tot = 0;
+    for (uint32_t w = 0; w < T / CPX_WAVE; w++) tot += (i128)red[w];
+    bd.parts[q.part_off + d.part] = tot;
+  }
+}
+
 template <int T, int K>
 // (at least 4 waves per SIMD: two 512-thread blocks per CU need <= 128 VGPRs)
 __global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
@@ -987,6 +1028,7 @@ __global__ __launch_bounds__(T, 4) void k_segments(BatchDev bd)
     if (ti >= ntiles) break;
     const TileDev tile = bd.tiles[ti];
     if (tile.group == CPX_TILE_EASY) easy_tile<T, K>(bd, (uint32_t)tile.seg, lds, sh);
+    else if (tile.group == CPX_TILE_P2) p2tab_tile<T, K>(bd, (uint32_t)tile.seg, lds);
     else hard_segment<T, K>(bd, tile, lds, sh);
     __syncthreads();
   }

@@ -124,38 +124,6 @@ hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s)
   return hipGetLastError();
 }
 
-// ------------------------------------------------------- P2 by lookups
-// Sum of pi(x/p) over a range of P2 primes p = primes[k] (y < p <= sqrt x,
-// x < 2^64), one block per descriptor: x/p by the reciprocal divide with the
-// tabulated 1/p (x/p <= x/(y+1) < 2^50), pi from the resident table, which
-// reaches x/(y+1).  The same sum the reference takes with upper_bound over
-// its window prime lists (src/P2.cu:76-133); k_combine subtracts
-// sum (pi(p) - 1) in closed form.
-__global__ __launch_bounds__(256) void k_p2tab(BatchDev bd)
-{
-  __shared__ i128 red[4];
-  const uint32_t nd = *bd.n_p2;
-  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
-    const BlockDesc d = bd.p2_desc[di];
-    const QueryDev &q = bd.queries[d.query];
-    const uint64_t x = q.x_lo;
-    const double xf = (double)x;
-    uint64_t s = 0;
-    for (uint64_t k = d.begin + threadIdx.x; k < d.end; k += blockDim.x) {
-      const uint64_t v = cpx_div_leaf_r(x, xf, bd.primes[k], bd.prinv[k]);
-      s += pi_lookup(bd.pitab, v);
-    }
-    const i128 tt = block_sum_i128((i128)s, red);
-    if (threadIdx.x == 0) bd.parts[q.part_off + d.part] = tt;
-  }
-}
-
-hipError_t launch_p2tab(BatchDev b, uint32_t grid, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_p2tab, dim3(grid), dim3(256), 0, s, b);
-  return hipGetLastError();
-}
-
 // -------------------------------------------------------- hard carry
 // Hard-leaf carry of stages [s0, s0 + ns) (ns <= CPX_FIN_STAGES) of one
 // query per block: for each of them
