@@ -470,7 +470,8 @@ def main():
     out["roofline"] = {"bound": "lds", "achieved": round(ach, 1), "peak": round(lds_peak / 1e9, 1),
                        "unit": "GB/s", "frac": round(ach / (lds_peak / 1e9), 4),
                        "traffic": traffic,
-                       "kernel": "k_segments (persistent hard-leaf + P2 tiles)",
+                       "kernel": "k_segments (persistent: hard-leaf + P2 segment tiles, easy-leaf and "
+                                 "P2-lookup tiles)",
                        "kernel_ms": round(seg_avg_s * 1000, 3),
                        "algorithmic_bytes": w_lds,
                        "peak_source": "measured on this device by cpx_probe_peaks (LDS 16-byte reads, all CUs)",
