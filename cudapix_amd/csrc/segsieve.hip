@@ -893,13 +893,12 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
 // over the descriptor's primes p = primes[j], j in [begin, end) (all these
 // leaves have x/pq < p^2).  The block sets up T primes at a time in parallel
 // (x/p, the q range, the (2 - pi(p)) terms), then strides its threads over
-// each prime's q's with U independent prime -> quotient -> pi-table chains
-// per thread (U * T = 1024 slots when T <= 256).  Uses the first 3T words of
-// the segment LDS.
+// each prime's q's with U = 4 independent prime -> quotient -> pi-table
+// chains per thread.  Uses the first 3T words of the segment LDS.
 template <int T, int K>
 __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint32_t *lds, SegShared<T, K> &sh)
 {
-  constexpr uint32_t U = T >= 512 ? 2 : 4;
+  constexpr uint32_t U = 4;
   uint64_t *sxp = reinterpret_cast<uint64_t *>(lds);    // x / p of the chunk's primes
   uint32_t *sk1 = lds + 2 * T;                          // pi(min(y, x / p^2)): q indices end here
   const uint32_t t = threadIdx.x;
