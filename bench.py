@@ -355,13 +355,18 @@ def main():
     ms_step = 1000.0 * dt / args.steps
     log("batched (pipelined): %.1f q/s, %.2f ms/step (device %.2f ms, segments %.2f ms)"
         % (qps, ms_step, sum(dev_ms) / len(dev_ms), sum(seg_ms) / len(seg_ms)))
-    # the same steps one call at a time (plan, upload, run, wait)
+    # the same steps one call at a time (plan, upload, run, wait); their
+    # segment-kernel times (no other batch beside it) feed the roofline
+    seg_ms = []
     t0 = time.time()
     for i in range(args.steps):
-        if ctx.pi_batch(xs, use_graph=True) != ref:
+        r, st = ctx.pi_batch(xs, use_graph=True, stats=True)
+        seg_ms.append(st["ms_segments"])
+        if r != ref:
             raise SystemExit("batched results changed between steps")
     dts = time.time() - t0
-    log("batched (one call at a time): %.1f q/s, %.2f ms/step" % (n * args.steps / dts, 1000.0 * dts / args.steps))
+    log("batched (one call at a time): %.1f q/s, %.2f ms/step (segments %.2f ms)"
+        % (n * args.steps / dts, 1000.0 * dts / args.steps, sum(seg_ms) / len(seg_ms)))
 
     # terms of the batch, for the roofline's algorithmic bytes
     terms = ctx.pi_batch(xs, terms=True)
