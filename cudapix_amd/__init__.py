@@ -454,12 +454,21 @@ class Context:
         """pi(x) of every batch in an iterable of x lists, pipelined two deep:
         yields each batch's results in order."""
         pending = []
-        for xs in batches:
-            pending.append(self.pi_batch_submit(xs, use_graph=use_graph, p2_table=p2_table))
-            if len(pending) == 2:
+        try:
+            for xs in batches:
+                pending.append(self.pi_batch_submit(xs, use_graph=use_graph, p2_table=p2_table))
+                if len(pending) == 2:
+                    yield self.pi_batch_collect(pending.pop(0))
+            while pending:
                 yield self.pi_batch_collect(pending.pop(0))
-        while pending:
-            yield self.pi_batch_collect(pending.pop(0))
+        finally:
+            # a consumer that stops early leaves batches in flight: collect
+            # them so that the context's slots are free again
+            for t in pending:
+                try:
+                    self.pi_batch_collect(t)
+                except CpxError:
+                    pass
 
     def autotune(self, xs):
         n = len(xs)

@@ -87,6 +87,13 @@ def _pipelined(C, ctx, oracle):
     batches = [b2, b1[:3], b3, b1, b2[:1]]
     assert list(ctx.pi_batches(batches)) == [ctx.pi_batch(b) for b in batches]
     assert list(ctx.pi_batches(batches, use_graph=False)) == [ctx.pi_batch(b) for b in batches]
+    # a consumer that stops after the first result leaves no batch in flight
+    gen = ctx.pi_batches(batches)
+    first = next(gen)                    # two submitted, the first collected
+    gen.close()                          # the generator collects the second
+    t1, t2 = ctx.pi_batch_submit(b3), ctx.pi_batch_submit(b1)   # both slots free again
+    assert ctx.pi_batch_collect(t1) == want[2] and ctx.pi_batch_collect(t2) == want[0]
+    assert first == ctx.pi_batch(batches[0])
     # a submission that grows the resident tables while the other slot's
     # batch is still queued (the old tables are freed only after it ran)
     big = [10 ** 16, 10 ** 16 + 1, 3 * 10 ** 15]
