@@ -986,6 +986,9 @@ static int seg_grid(cpx_ctx *c)
 }
 
 static const uint32_t GRID_Q = 256, GRID_T = 2048;
+// carry kernel: 128-thread blocks whose time is a latency chain, so many of
+// them in flight (extra blocks of a small batch exit at once)
+static const uint32_t GRID_FIN = 16384;
 
 // The launch sequence in three parts; events between the parts time the
 // persistent segment kernel on its own stream (also when replayed as graphs).
@@ -1004,7 +1007,7 @@ static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_w
   } else if (part == 1) {
     HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
   } else {
-    HIPCHK(launch_hard_finalize(b, GRID_T, c->stream));
+    HIPCHK(launch_hard_finalize(b, GRID_FIN, c->stream));
     HIPCHK(launch_combine(b, GRID_Q, c->stream));
   }
   return CPX_OK;

@@ -131,15 +131,16 @@ hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s)
 //             msum_s(r) * (T_s(0) + ... + T_s(r-1))
 // with msum_s(r) the replicas' sum(mu) of the leaves of p_s in segment r and
 // T_s(r) its survivors before stage s.  Activity is a prefix of the segments
-// (bact is non-increasing in r), so the block scans 256 segments at a time
+// (bact is non-increasing in r), so the block scans FIN_T segments at a time
 // and stops after the first chunk that ends inactive for s0.  A thread reads
 // its segment's ns consecutive T and sum(mu) entries, which share cache lines
 // (the row record is read once for all of them).
-__global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
+#define FIN_T 128          // carry-block threads: segments per chunk
+__global__ __launch_bounds__(FIN_T) void k_hard_fin(BatchDev bd)
 {
-  constexpr uint32_t FS = CPX_FIN_STAGES;
-  __shared__ i128 red[4];
-  __shared__ unsigned long long wt[FS][4];
+  constexpr uint32_t FS = CPX_FIN_STAGES, NW = FIN_T / CPX_WAVE;
+  __shared__ i128 red[NW];
+  __shared__ unsigned long long wt[FS][NW];
   __shared__ uint32_t stop;
   const uint32_t nd = *bd.n_fin;
   const uint32_t t = threadIdx.x;
@@ -163,10 +164,10 @@ __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
       }
     };
     fetch(t);
-    for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += 256) {
+    for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += FIN_T) {
       const uint32_t ba = ba_n;
       const HardRowInfo ri = ri_n;
-      fetch(r0 + 256 + t);
+      fetch(r0 + FIN_T + t);
       unsigned long long tv[FS], incl[FS];
       long long msum[FS];
 #pragma unroll
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
       for (uint32_t j = 0; j < FS; j++) {
         if (j < ns) {
           unsigned long long pre = incl[j] - tv[j], tot = 0;
-          for (uint32_t w = 0; w < 4; w++) {
+          for (uint32_t w = 0; w < NW; w++) {
             tot += wt[j][w];
             if (w < wave_id()) pre += wt[j][w];
           }
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
           base[j] += tot;
         }
       }
-      if (t == 255) stop = act ? 0u : 1u;
+      if (t == FIN_T - 1) stop = act ? 0u : 1u;
       __syncthreads();
       if (stop) break;
     }
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(256) void k_hard_fin(BatchDev bd)
 
 hipError_t launch_hard_finalize(BatchDev b, uint32_t grid, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_hard_fin, dim3(grid), dim3(256), 0, s, b);
+  hipLaunchKernelGGL(k_hard_fin, dim3(grid), dim3(FIN_T), 0, s, b);
   return hipGetLastError();
 }
 
