@@ -899,6 +899,7 @@ template <int T, int K>
 __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint32_t *lds, SegShared<T, K> &sh)
 {
   constexpr uint32_t U = 4;
+  constexpr uint32_t SBT = T < 256 ? T : 256, SUB = T / SBT;
   uint64_t *sxp = reinterpret_cast<uint64_t *>(lds);    // x / p of the chunk's primes
   uint32_t *sk1 = lds + 2 * T;                          // pi(min(y, x / p^2)): q indices end here
   const uint32_t t = threadIdx.x;
@@ -923,21 +924,24 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
       if (k1 > j + 1) sum += (i128)(k1 - j - 1) * (1 - (i128)j);   // (2 - pi(p)) per q, pi(p) = j + 1
     }
     __syncthreads();
-    for (uint32_t jj = 0; jj < nj; jj++) {
+    // sub-blocks of SBT threads take every SUB-th prime, so that a prime's
+    // q's fill U * SBT slots per step (1024 at 256 threads or more: fewer
+    // idle lanes in a prime's last step)
+    for (uint32_t jj = t / SBT; jj < nj; jj += SUB) {
       const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
       if (k1 <= j + 1) continue;
       // a chain past k1 repeats index k and is masked; x/(p q) < x/p^2 <
       // 2^50, so the reciprocal divide with the tabulated 1/q is exact
       const double xpf = (double)xp;
       uint64_t s = 0;
-      for (uint64_t k = j + 1 + t; k < k1; k += U * T) {
+      for (uint64_t k = j + 1 + t % SBT; k < k1; k += U * SBT) {
         uint32_t qv[U];
         double rv[U];
         bool ok[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-          ok[u] = k + u * T < k1;
-          const uint64_t kk = ok[u] ? k + u * T : k;
+          ok[u] = k + u * SBT < k1;
+          const uint64_t kk = ok[u] ? k + u * SBT : k;
           qv[u] = bd.primes[kk];
           rv[u] = bd.prinv[kk];
         }
