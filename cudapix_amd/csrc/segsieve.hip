@@ -899,7 +899,7 @@ template <int T, int K>
 __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint32_t *lds, SegShared<T, K> &sh)
 {
   constexpr uint32_t U = 4;
-  constexpr uint32_t SBT = T < 256 ? T : 256, SUB = T / SBT;
+  constexpr uint32_t SBT = T < 256 ? T : 256, SUB = T / SBT, BIG = 8 * U * T;
   uint64_t *sxp = reinterpret_cast<uint64_t *>(lds);    // x / p of the chunk's primes
   uint32_t *sk1 = lds + 2 * T;                          // pi(min(y, x / p^2)): q indices end here
   const uint32_t t = threadIdx.x;
@@ -924,42 +924,46 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
       if (k1 > j + 1) sum += (i128)(k1 - j - 1) * (1 - (i128)j);   // (2 - pi(p)) per q, pi(p) = j + 1
     }
     __syncthreads();
-    // sub-blocks of SBT threads take every SUB-th prime, so that a prime's
-    // q's fill U * SBT slots per step (1024 at 256 threads or more: fewer
-    // idle lanes in a prime's last step)
-    for (uint32_t jj = t / SBT; jj < nj; jj += SUB) {
-      const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
-      if (k1 <= j + 1) continue;
-      // a chain past k1 repeats index k and is masked; x/(p q) < x/p^2 <
-      // 2^50, so the reciprocal divide with the tabulated 1/q is exact
-      const double xpf = (double)xp;
-      uint64_t s = 0;
-      for (uint64_t k = j + 1 + t % SBT; k < k1; k += U * SBT) {
-        uint32_t qv[U];
-        double rv[U];
-        bool ok[U];
+    // primes with at least BIG q's take the whole block one at a time (U * T
+    // slots per step); the others go to sub-blocks of SBT threads, every SUB-th
+    // prime to one sub-block (U * SBT slots per step: fewer idle lanes in a
+    // prime's last step)
+    for (uint32_t pass = 0; pass < 2; pass++) {
+      const uint32_t nthr = pass ? SBT : T, stride = U * nthr, tl = pass ? t % SBT : t;
+      for (uint32_t jj = pass ? t / SBT : 0; jj < nj; jj += pass ? SUB : 1) {
+        const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
+        if (k1 <= j + 1 || ((k1 - j - 1 >= BIG) != (pass == 0))) continue;
+        // a chain past k1 repeats index k and is masked; x/(p q) < x/p^2 <
+        // 2^50, so the reciprocal divide with the tabulated 1/q is exact
+        const double xpf = (double)xp;
+        uint64_t s = 0;
+        for (uint64_t k = j + 1 + tl; k < k1; k += stride) {
+          uint32_t qv[U];
+          double rv[U];
+          bool ok[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-          ok[u] = k + u * SBT < k1;
-          const uint64_t kk = ok[u] ? k + u * SBT : k;
-          qv[u] = bd.primes[kk];
-          rv[u] = bd.prinv[kk];
-        }
-        PiEntry pe[U];
-        uint32_t ix[U];
+          for (uint32_t u = 0; u < U; u++) {
+            ok[u] = k + u * nthr < k1;
+            const uint64_t kk = ok[u] ? k + u * nthr : k;
+            qv[u] = bd.primes[kk];
+            rv[u] = bd.prinv[kk];
+          }
+          PiEntry pe[U];
+          uint32_t ix[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-          const uint64_t v = cpx_div_leaf_r(xp, xpf, qv[u], rv[u]);
-          ix[u] = (uint32_t)(((v & 63) + 1) >> 1) | (v >= 2 ? 0x80u : 0u);
-          pe[u] = bd.pitab[v >> 6];
-        }
+          for (uint32_t u = 0; u < U; u++) {
+            const uint64_t v = cpx_div_leaf_r(xp, xpf, qv[u], rv[u]);
+            ix[u] = (uint32_t)(((v & 63) + 1) >> 1) | (v >= 2 ? 0x80u : 0u);
+            pe[u] = bd.pitab[v >> 6];
+          }
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-          const uint32_t c = (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
-          s += ok[u] ? c : 0u;
+          for (uint32_t u = 0; u < U; u++) {
+            const uint32_t c = (ix[u] >> 7) + pe[u].count + cpx_popc32(pe[u].bits & cpx_lowmask32(ix[u] & 63));
+            s += ok[u] ? c : 0u;
+          }
         }
+        sum += (i128)s;
       }
-      sum += (i128)s;
     }
     __syncthreads();
   }
