@@ -303,8 +303,8 @@ def main():
     if args.serve_full:
         args.serve_lambdas, args.serve_n, args.serve_xmax = "50,200,500,1000,2000", 2000, 1e17
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here has a single MI355X and this "
-              "workload shards by query with no exchange; run with --gpus 1",
+        print("bench.py measures one GPU only (the workload shards by query with no exchange); "
+              "run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
