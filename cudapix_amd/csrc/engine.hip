@@ -290,7 +290,7 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
-static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 65536, P2_CH = 65536;
+static const uint64_t ORD_CH = 8192, TRIV_CH = 4096, EASY_WORK = 131072, P2_CH = 65536;
 
 static void plan_needs(const cpx_ctx *c, const HostQuery &h, uint32_t mask, uint64_t SB,
                        uint64_t *R, uint64_t *P, uint64_t *Y)
