@@ -58,20 +58,22 @@ class CpxBatchStats(ctypes.Structure):
 
 class CpxServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", U64), ("max_batch", ctypes.c_uint32),
-                ("max_wait_ms", ctypes.c_double)]
+                ("max_wait_ms", ctypes.c_double), ("slots", ctypes.c_uint32), ("max_x", CpxU128)]
 
 
 class CpxResult(ctypes.Structure):
     _fields_ = [("ticket", U64), ("x", CpxU128), ("pi", CpxU128), ("status", ctypes.c_int),
                 ("t_arrive_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double),
-                ("t_done_ms", ctypes.c_double)]
+                ("t_done_ms", ctypes.c_double), ("launch", U64), ("cohort", U64),
+                ("dev_first_ms", ctypes.c_double), ("dev_last_ms", ctypes.c_double)]
 
 
 class CpxServerStats(ctypes.Structure):
     _fields_ = [("submitted", U64), ("completed", U64), ("rejected", U64), ("cancelled", U64),
                 ("batches", U64), ("max_inflight_bytes", U64), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
-                ("qps", ctypes.c_double)]
+                ("qps", ctypes.c_double), ("launches", U64), ("joined", U64),
+                ("arena_bytes", U64)]
 
 
 # every symbol include/cudapix.h declares (checked by the CPU tests)
@@ -305,6 +307,36 @@ class Context:
         self._chk(self._L.cpx_v_split_u64(self.h, x, y, ctypes.byref(a), ctypes.byref(b)),
                   "V_split")
         return a.value, b.value
+
+    def V_a(self, x, y):
+        """V over x^(1/4) < p <= sqrt(x/y) (include/V.cuh:7)."""
+        return self.V_split(x, y)[0]
+
+    def V_b(self, x, y):
+        """V over sqrt(x/y) < p <= cbrt(x) (include/V.cuh:8)."""
+        return self.V_split(x, y)[1]
+
+    def terms_u64(self, x, y, c=6):
+        """Every term through the 64-bit entry points, as the reference's
+        64-bit callers bind them (include/P2.cuh:16, ordinary.cuh:6,
+        trivial.cuh:8, S3.cuh:3, V.cuh:6, deleglise-rivat.cpp:50)."""
+        L, h = self._L, self.h
+        u, i = U64(), ctypes.c_int64()
+        out = {}
+        self._chk(L.cpx_p2_u64(h, x, y, ctypes.byref(u)), "P2 u64")
+        out["p2"] = u.value
+        self._chk(L.cpx_ordinary_u64(h, x, y, c, ctypes.byref(i)), "Ordinary u64")
+        out["ordinary"] = i.value
+        self._chk(L.cpx_s2_trivial_u64(h, x, y, ctypes.byref(u)), "S2_trivial u64")
+        out["trivial"] = u.value
+        self._chk(L.cpx_s3_u64(h, x, y, c, ctypes.byref(i)), "S3 u64")
+        out["hard"] = i.value
+        self._chk(L.cpx_v_u64(h, x, y, ctypes.byref(i)), "V u64")
+        out["easy"] = i.value
+        t = CpxTerms()
+        self._chk(L.cpx_pi_u64(h, x, y, c, ctypes.byref(u), ctypes.byref(t)), "pi u64")
+        out["pi"] = u.value
+        return out
 
     # ---- primes / tables
     def countPrimes(self, lo, hi):
@@ -574,11 +606,12 @@ class Phi:
 
 
 class Server:
-    """Continuous-batching pi(x) server (admission by predicted HBM bytes)."""
+    """Continuous-batching pi(x) server: requests join the running streaming
+    launch as cohorts, admitted by the HBM bytes of their planned state."""
 
-    def __init__(self, ctx, hbm_budget=0, max_batch=256, max_wait_ms=0.5):
+    def __init__(self, ctx, hbm_budget=0, max_batch=256, max_wait_ms=2.0, slots=4, max_x=0):
         self.ctx = ctx
-        o = CpxServerOpts(hbm_budget, max_batch, max_wait_ms)
+        o = CpxServerOpts(hbm_budget, max_batch, max_wait_ms, slots, u128(max_x))
         h = ctypes.c_void_p()
         ctx._chk(ctx._L.cpx_server_create(ctx.h, ctypes.byref(o), ctypes.byref(h)), "Server")
         self.h = h
@@ -613,7 +646,9 @@ class Server:
         self._L.cpx_server_poll(self.h, out, max_, ctypes.byref(n))
         return [{"ticket": r.ticket, "x": from_u128(r.x), "pi": from_u128(r.pi),
                  "status": r.status, "t_arrive_ms": r.t_arrive_ms, "t_admit_ms": r.t_admit_ms,
-                 "t_done_ms": r.t_done_ms} for r in out[:n.value]]
+                 "t_done_ms": r.t_done_ms, "launch": r.launch, "cohort": r.cohort,
+                 "dev_first_ms": r.dev_first_ms, "dev_last_ms": r.dev_last_ms}
+                for r in out[:n.value]]
 
     def stats(self):
         s = CpxServerStats()

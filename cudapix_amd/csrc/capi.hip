@@ -926,9 +926,14 @@ int cpx_get_tuning(cpx_ctx *c, uint32_t *threads, uint32_t *kwords, double *alph
 uint64_t cpx_request_bytes(cpx_ctx *c, cpx_u128 x)
 {
   if (!c) return 0;
+  std::lock_guard<std::recursive_mutex> g(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return 0;
   HostQuery h;
   if (make_host_query(c, to_u128(x), 0, 0, &h)) return 0;
-  return plan_bytes_estimate(c, h);
+  std::vector<HostQuery> v(1, h);
+  uint64_t b = 0;
+  if (stream_need_bytes(c, v, &b)) return 0;
+  return b;
 }
 
 // ------------------------------------------------------------ selftests

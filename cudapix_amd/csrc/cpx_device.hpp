@@ -116,6 +116,35 @@ __device__ __forceinline__ i128 block_sum_i128(i128 v, i128 *scratch)
   return t;   // valid in thread 0
 }
 
+// Output rows that another block reads in the same launch (the streaming
+// kernel's carry and combine units read what segment tiles wrote): stored
+// with device-scope coherence, i.e. written through to the point every XCD's
+// L2 reads from, so that no agent-scope fence (a whole-L2 write-back and
+// invalidate on this device) is needed between producer and consumer.  The
+// producer still waits for its stores (stream_release) before it signals.
+__device__ __forceinline__ void st_out(uint32_t *p, uint32_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_out(int32_t *p, int32_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_out(uint64_t *p, uint64_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_out(int64_t *p, int64_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_out(i128 *p, i128 v)
+{
+  uint64_t *q = reinterpret_cast<uint64_t *>(p);
+  st_out(q, (uint64_t)v);
+  st_out(q + 1, (uint64_t)((u128)v >> 64));
+}
+
 // Padded-linear LDS layout of a segment bitmap: segment word g lives at
 // lds[g + g / K].  Thread t owns the contiguous words [t*K, (t+1)*K); its
 // j-th word sits at lds[t*(K+1) + j], so lanes walking their own words in

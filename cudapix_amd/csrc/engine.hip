@@ -229,15 +229,31 @@ int build_wheels(cpx_ctx *c, uint32_t want)
   return CPX_OK;
 }
 
+// The table limits that needs (R: pi table, P: prime list, Y: mu*lpf) ask for.
+static void table_limits(uint64_t R, uint64_t P, uint64_t Y, uint64_t *Rm_, uint64_t *Pm_, uint64_t *Ym_)
+{
+  if (Y < 64) Y = 64;
+  uint64_t Pm = std::max<uint64_t>(P, cpx_isqrt128((u128)Y + 16384) + 1);
+  Pm = std::max<uint64_t>(Pm, Y);
+  *Rm_ = std::max<uint64_t>({R, Pm, (uint64_t)1 << 14});
+  *Pm_ = Pm;
+  *Ym_ = Y;
+}
+
+bool tables_cover(const cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
+{
+  uint64_t Rm, Pm, Ym;
+  table_limits(R, P, Y, &Rm, &Pm, &Ym);
+  return c->small.p && c->pitab.p && c->pitab_limit >= Rm && c->prime_limit >= Pm && c->mlpf_limit >= Ym;
+}
+
 int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 {
   int rc;
   if ((rc = build_bootstrap(c))) return rc;
   if ((rc = build_wheels(c, 0))) return rc;
-  if (Y < 64) Y = 64;
-  uint64_t Pm = std::max<uint64_t>(P, cpx_isqrt128((u128)Y + 16384) + 1);
-  Pm = std::max<uint64_t>(Pm, Y);
-  uint64_t Rm = std::max<uint64_t>({R, Pm, (uint64_t)1 << 14});
+  uint64_t Rm, Pm;
+  table_limits(R, P, Y, &Rm, &Pm, &Y);
   if (!c->opts.fresh_tables && c->pitab_limit >= Rm && c->prime_limit >= Pm && c->mlpf_limit >= Y)
     return CPX_OK;
   if (!c->opts.fresh_tables) {                      // headroom for later, larger queries
@@ -344,6 +360,10 @@ struct Plan {
   std::vector<QPlan> qps;
   std::vector<PlanOff> off;
   std::vector<BlockDesc> fin;        // carry blocks of all queries, longest first
+  bool stream = false;               // a streaming cohort: ordinary and trivial as tiles,
+                                     // stage table from the host
+  Caps base{};                       // where its entries start in each arena array (a
+                                     // streaming cohort's slot; zero for a batch)
   size_t nord = 0, ntriv = 0, neasy = 0, nfin = 0, np2d = 0, nrows = 0;
   uint64_t stage_total = 0, hard_rows = 0, hard_tm = 0, p2_rows = 0, parts = 0;
   uint64_t hard_mrows = 0, hard_mm = 0;
@@ -369,6 +389,7 @@ static inline uint16_t cost_bucket(double cost)
 }
 
 static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_t mask, QPlan &qp)
+// (i: the query's index in the arena's query array, which its descriptors name)
 {
   const uint64_t SB = c->shape.seg_bits();
   const uint32_t T = (uint32_t)c->shape.threads;
@@ -700,13 +721,17 @@ static unsigned plan_threads(size_t nq)
   return std::min<unsigned>(hc ? hc : 4, 16);
 }
 
-static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl)
+static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl,
+                      bool stream = false, const Caps *base = nullptr)
 {
+  pl.stream = stream;
+  pl.base = base ? *base : Caps{};
+  const Caps &B = pl.base;
   const size_t nq = hq.size();
   std::vector<QPlan> &qps = pl.qps;
   qps.assign(nq, QPlan());
   const unsigned nt = plan_threads(nq);
-  parallel_for(nq, nt, [&](size_t i) { plan_query(c, hq[i], (uint32_t)i, mask, qps[i]); });
+  parallel_for(nq, nt, [&](size_t i) { plan_query(c, hq[i], (uint32_t)(B.q + i), mask, qps[i]); });
   pl.groups.clear();
   make_groups(c, hq, qps, pl.groups);
   std::vector<GPlan> gps(pl.groups.size());
@@ -720,13 +745,13 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   for (size_t i = 0; i < nq; i++) {
     QPlan &qp = qps[i];
     QueryDev &d = qp.d;
-    d.part_off = pl.parts;
-    d.stage_off = pl.stage_total;
-    d.hard_row0 = pl.hard_rows;
-    d.tm_off = pl.hard_tm;
-    d.p2_row0 = pl.p2_rows;
-    d.mrow_base = pl.hard_mrows;
-    d.m_off = pl.hard_mm;
+    d.part_off = B.parts + pl.parts;
+    d.stage_off = B.stages + pl.stage_total;
+    d.hard_row0 = B.rows + pl.hard_rows;
+    d.tm_off = B.tm + pl.hard_tm;
+    d.p2_row0 = B.p2rows + pl.p2_rows;
+    d.mrow_base = B.mrows + pl.hard_mrows;
+    d.m_off = B.mm + pl.hard_mm;
     pl.parts += qp.parts;
     pl.stage_total += d.nstage;
     pl.hard_rows += d.hard_nseg;
@@ -768,6 +793,13 @@ static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mas
   };
   spread(CPX_TILE_EASY, pl.neasy);
   spread(CPX_TILE_P2, pl.np2d);
+  if (stream) {
+    spread(CPX_TILE_ORD, pl.nord);
+    spread(CPX_TILE_TRIV, pl.ntriv);
+    // a cohort of table lookups only still needs one main tile: its end
+    // opens the combine phase
+    if (pl.tiles.empty()) pl.tiles.push_back({CPX_TILE_NOP, 1, 0, 0});
+  }
   // carry blocks longest first too (a block's time is its chain of row
   // chunks), so the grid-stride loop does not end on a few long ones
   std::fill(start.begin(), start.end(), 0u);
@@ -826,9 +858,9 @@ static Caps needed_caps(const Plan &pl)
   return k;
 }
 
-static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
+static int ensure_arena_caps(cpx_ctx *c, BatchSlot &S, const Caps &need, uint64_t budget)
 {
-  Caps need = needed_caps(pl);
+  const Caps before = S.caps;
   Caps &k = S.caps;
   bool grow = !S.d_plan || !S.d_out;
   auto fit = [&](uint64_t &cap, uint64_t n) {
@@ -842,6 +874,15 @@ static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
   fit(k.mm, need.mm); fit(k.mrows, need.mrows);
   if (!grow) return CPX_OK;
   Layout L = make_layout(k);
+  if (budget && L.plan_bytes + L.out_bytes > budget) {
+    // without the growth headroom, if that fits
+    uint64_t *kk = &k.q;
+    const uint64_t *nn = &need.q, *oo = &before.q;
+    for (size_t f = 0; f < sizeof(Caps) / sizeof(uint64_t); f++) kk[f] = std::max(nn[f], oo[f]);
+    k.q = std::max<uint64_t>(k.q, 64);
+    k.groups = std::max<uint64_t>(k.groups, 16);
+    L = make_layout(k);
+  }
   // this slot's last batch was collected; the other slots' buffers are untouched
   if (S.d_plan) (void)hipFree(S.d_plan);
   if (S.d_out) (void)hipFree(S.d_out);
@@ -852,9 +893,8 @@ static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
   S.arena_gen++;
   const Caps kk = k;
   auto fail = [&](int rc) { S.caps = Caps{}; return rc; };
-  uint64_t budget = c->opts.hbm_budget;
-  if (budget && c->table_bytes + slot_bytes(c) + L.plan_bytes + L.out_bytes > budget)
-    return fail(ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget"));
+  if (budget && L.plan_bytes + L.out_bytes > budget)
+    return fail(ctx_fail(c, CPX_ENOMEM, "batch state exceeds the HBM budget"));
   hipError_t e;
   if ((e = hipMalloc(&S.d_plan, L.plan_bytes)) != hipSuccess) { S.d_plan = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(plan)")); }
   if ((e = hipMalloc(&S.d_out, L.out_bytes)) != hipSuccess) { S.d_out = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(out)")); }
@@ -862,6 +902,16 @@ static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
   if ((e = hipHostMalloc(&S.h_res, kk.q * sizeof(ResultDev), hipHostMallocDefault)) != hipSuccess) { S.h_res = nullptr; return fail(hip_fail(c, e, "hipHostMalloc")); }
   S.L = L;
   return CPX_OK;
+}
+
+static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
+{
+  const uint64_t budget = c->opts.hbm_budget;
+  uint64_t other = c->table_bytes + slot_bytes(c);
+  if (S.d_plan && S.d_out) other -= S.L.plan_bytes + S.L.out_bytes;
+  if (budget && other >= budget)
+    return ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget");
+  return ensure_arena_caps(c, S, needed_caps(pl), budget ? budget - other : 0);
 }
 
 uint64_t slot_bytes(const cpx_ctx *c)
@@ -922,12 +972,9 @@ static BatchDev make_batchdev(cpx_ctx *c, const BatchSlot &S)
   return b;
 }
 
-// Writes the plan into the slot's pinned staging and queues its copy on the
-// upload stream; the batch stream waits for it (event ev[4]).
-static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
+// Writes the plan into a host image of the plan arena (layout L).
+static void pack_plan(const Layout &L, const Plan &pl, char *H)
 {
-  const Layout &L = S.L;
-  char *H = (char *)S.h_plan;
   uint32_t *cnt = (uint32_t *)(H + L.counters);
   memset(cnt, 0, 64);
   cnt[0] = (uint32_t)pl.q.size(); cnt[1] = (uint32_t)pl.tiles.size(); cnt[2] = 0;
@@ -937,20 +984,71 @@ static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
   auto put = [&](size_t off, const void *src, size_t bytes) {
     if (bytes) memcpy(H + off, src, bytes);
   };
-  put(L.queries, pl.q.data(), pl.q.size() * sizeof(QueryDev));
-  put(L.tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDev));
-  put(L.groups, pl.groups.data(), pl.groups.size() * sizeof(GroupDev));
-  put(L.fin, pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
+  const Caps &B = pl.base;
+  put(L.queries + B.q * sizeof(QueryDev), pl.q.data(), pl.q.size() * sizeof(QueryDev));
+  put(L.fin + B.fin * sizeof(BlockDesc), pl.fin.data(), pl.fin.size() * sizeof(BlockDesc));
+  // tiles and groups name groups, descriptors and queries: rebased to the
+  // cohort's entries (a batch's base is zero)
+  TileDev *td = (TileDev *)(H + L.tiles) + B.tiles;
+  for (size_t k = 0; k < pl.tiles.size(); k++) {
+    TileDev t = pl.tiles[k];
+    switch (t.group) {
+      case CPX_TILE_EASY: t.seg += B.easy; break;
+      case CPX_TILE_P2: t.seg += B.p2d; break;
+      case CPX_TILE_ORD: t.seg += B.ord; break;
+      case CPX_TILE_TRIV: t.seg += B.triv; break;
+      case CPX_TILE_NOP: break;
+      default: t.group += (uint32_t)B.groups; break;
+    }
+    td[k] = t;
+  }
+  GroupDev *gd = (GroupDev *)(H + L.groups) + B.groups;
+  for (size_t k = 0; k < pl.groups.size(); k++) {
+    GroupDev g = pl.groups[k];
+    for (uint32_t m = 0; m < g.n; m++) g.q[m] += (uint32_t)B.q;
+    gd[k] = g;
+  }
   // every query's descriptors and rows straight to their places
   parallel_for(pl.qps.size(), plan_threads(pl.qps.size()), [&](size_t i) {
     const QPlan &qp = pl.qps[i];
     const PlanOff &o = pl.off[i];
-    put(L.ord + o.ord * sizeof(BlockDesc), qp.ord.data(), qp.ord.size() * sizeof(BlockDesc));
-    put(L.triv + o.triv * sizeof(BlockDesc), qp.triv.data(), qp.triv.size() * sizeof(BlockDesc));
-    put(L.easy + o.easy * sizeof(BlockDesc), qp.easy.data(), qp.easy.size() * sizeof(BlockDesc));
-    put(L.p2d + o.p2d * sizeof(BlockDesc), qp.p2d.data(), qp.p2d.size() * sizeof(BlockDesc));
-    put(L.rowinfo + o.rows * sizeof(HardRowInfo), qp.rowinfo.data(), qp.rowinfo.size() * sizeof(HardRowInfo));
+    put(L.ord + (B.ord + o.ord) * sizeof(BlockDesc), qp.ord.data(), qp.ord.size() * sizeof(BlockDesc));
+    put(L.triv + (B.triv + o.triv) * sizeof(BlockDesc), qp.triv.data(), qp.triv.size() * sizeof(BlockDesc));
+    put(L.easy + (B.easy + o.easy) * sizeof(BlockDesc), qp.easy.data(), qp.easy.size() * sizeof(BlockDesc));
+    put(L.p2d + (B.p2d + o.p2d) * sizeof(BlockDesc), qp.p2d.data(), qp.p2d.size() * sizeof(BlockDesc));
+    put(L.rowinfo + (B.rows + o.rows) * sizeof(HardRowInfo), qp.rowinfo.data(),
+        qp.rowinfo.size() * sizeof(HardRowInfo));
   });
+}
+
+// The stage table of a streaming cohort, as k_setup writes it for a batch:
+// {floor(x / p), 1 / p, ceil(2^64 / p), p, floor(y / p)} for the hard stages.
+static void pack_stages(const cpx_ctx *c, const Layout &L, const Plan &pl, char *H)
+{
+  StageDev *st = (StageDev *)(H + L.stages);
+  parallel_for(pl.q.size(), plan_threads(pl.q.size()), [&](size_t i) {
+    const QueryDev &q = pl.q[i];
+    const u128 x = ((u128)q.x_hi << 64) | q.x_lo;
+    for (uint32_t k = 0; k < q.nstage; k++) {
+      const uint32_t p = c->h_primes[q.c + k];
+      StageDev d;
+      d.p = p;
+      d.ylo = (uint32_t)(q.y / p);
+      d.inv_p = 1.0 / (double)p;
+      d.magic = ~0ull / p + 1;
+      d.xp = (uint64_t)(x / p);
+      st[q.stage_off + k] = d;
+    }
+  });
+}
+
+// Writes the plan into the slot's pinned staging and queues its copy on the
+// upload stream; the batch stream waits for it (event ev[4]).
+static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
+{
+  const Layout &L = S.L;
+  char *H = (char *)S.h_plan;
+  pack_plan(L, pl, H);
   // copy only the used prefix of each array
   auto up = [&](size_t off, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
@@ -1197,4 +1295,148 @@ int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uin
   const int rc = submit_queries(c, hq, mask, p2mode, use_graph, &si);
   if (rc) return rc;
   return collect_queries(c, si, res, st);
+}
+
+// ------------------------------------------------------ streaming cohorts
+
+// Table needs of a cohort (P2 from the resident pi table where it reaches,
+// as every streaming cohort takes it).
+void stream_table_needs(const cpx_ctx *c, std::vector<HostQuery> &hq, uint64_t *R, uint64_t *P, uint64_t *Y,
+                        bool p2_lookups)
+{
+  choose_p2tab(c, hq, TERM_ALL, p2_lookups ? 0 : 1);
+  if (p2_lookups)
+    for (HostQuery &h : hq)                    // every eligible query, even alone
+      h.p2tab = !h.small && h.y < h.sqrtx && h.sqrtx < (1ull << 32) && h.zt < P2TAB_LIMIT;
+  *R = *P = *Y = 0;
+  for (const HostQuery &h : hq) plan_needs(c, h, TERM_ALL, c->shape.seg_bits(), R, P, Y);
+}
+
+int stream_plan(cpx_ctx *c, std::vector<HostQuery> &hq, BatchSlot &S, const Caps &cursor, bool grow,
+                uint64_t budget, CohortInfo *ci, double mult)
+{
+  *ci = CohortInfo{};
+  uint64_t R, P, Y;
+  stream_table_needs(c, hq, &R, &P, &Y);
+  uint32_t cmax = 1;
+  for (const HostQuery &h : hq) cmax = std::max(cmax, h.c);
+  if (!tables_cover(c, R, P, Y) || cmax > c->wheel_c) {
+    ci->why = CohortInfo::TABLES;
+    return CPX_EAGAIN;
+  }
+  const size_t NF = sizeof(Caps) / sizeof(uint64_t);
+  // the cohort's entries start at the cursor, rounded up to CPX_STREAM_ALIGN
+  Caps base = cursor;
+  {
+    uint64_t *b = &base.q;
+    for (size_t f = 0; f < NF; f++) b[f] = (b[f] + CPX_STREAM_ALIGN - 1) / CPX_STREAM_ALIGN * CPX_STREAM_ALIGN;
+  }
+  Plan pl;
+  int rc = build_plan(c, hq, TERM_ALL, pl, true, &base);
+  if (rc) return rc;
+  const Caps need = needed_caps(pl);
+  const Layout Ln = make_layout(need);
+  ci->need_bytes = Ln.plan_bytes + Ln.out_bytes;
+  if (budget && ci->need_bytes > budget) {
+    ci->why = CohortInfo::TOO_BIG;
+    return ctx_fail(c, CPX_ENOMEM, "cohort state exceeds the server's HBM budget");
+  }
+  auto fits = [&](const Caps &b) {
+    if (!S.d_plan || !S.d_out) return false;
+    const uint64_t *bb = &b.q, *n = &need.q, *k = &S.caps.q;
+    for (size_t f = 0; f < NF; f++)
+      if (bb[f] + n[f] > k[f]) return false;
+    return true;
+  };
+  if (!fits(base)) {
+    if (!grow) {
+      ci->why = CohortInfo::CAPS;
+      return CPX_EAGAIN;
+    }
+    // an empty arena that holds mult cohorts like this one (within the budget)
+    Caps want{};
+    {
+      uint64_t *w = &want.q;
+      const uint64_t *n = &need.q, *k = &S.caps.q;
+      for (size_t f = 0; f < NF; f++)
+        w[f] = std::max(k[f], (uint64_t)((n[f] + CPX_STREAM_ALIGN) * std::max(1.0, mult)));
+    }
+    if ((rc = ensure_arena_caps(c, S, want, budget))) {
+      ci->why = CohortInfo::TOO_BIG;
+      return rc;
+    }
+    base = Caps{};
+    if (!fits(base)) {
+      ci->why = CohortInfo::TOO_BIG;
+      return ctx_fail(c, CPX_ENOMEM, "cohort does not fit the arena");
+    }
+    pl = Plan();
+    if ((rc = build_plan(c, hq, TERM_ALL, pl, true, &base))) return rc;
+  }
+  const Layout &L = S.L;
+  char *H = (char *)S.h_plan;
+  pack_plan(L, pl, H);
+  pack_stages(c, L, pl, H);
+  ci->nq = (uint32_t)pl.q.size();
+  ci->n_main = (uint32_t)pl.tiles.size();
+  ci->n_fin = (uint32_t)pl.nfin;
+  ci->q_base = base.q;
+  ci->tile_base = base.tiles;
+  ci->fin_base = base.fin;
+  {
+    uint64_t *nx = &ci->next.q;
+    const uint64_t *b = &base.q, *n = &need.q;
+    for (size_t f = 0; f < NF; f++) nx[f] = b[f] + n[f];
+  }
+  // the cohort's entries of every plan array (the pinned image has the arena's layout)
+  const size_t rg[][2] = {{L.queries + base.q * sizeof(QueryDev), pl.q.size() * sizeof(QueryDev)},
+                          {L.stages + base.stages * sizeof(StageDev), pl.stage_total * sizeof(StageDev)},
+                          {L.tiles + base.tiles * sizeof(TileDev), pl.tiles.size() * sizeof(TileDev)},
+                          {L.groups + base.groups * sizeof(GroupDev), pl.groups.size() * sizeof(GroupDev)},
+                          {L.ord + base.ord * sizeof(BlockDesc), pl.nord * sizeof(BlockDesc)},
+                          {L.triv + base.triv * sizeof(BlockDesc), pl.ntriv * sizeof(BlockDesc)},
+                          {L.easy + base.easy * sizeof(BlockDesc), pl.neasy * sizeof(BlockDesc)},
+                          {L.fin + base.fin * sizeof(BlockDesc), pl.nfin * sizeof(BlockDesc)},
+                          {L.p2d + base.p2d * sizeof(BlockDesc), pl.np2d * sizeof(BlockDesc)},
+                          {L.rowinfo + base.rows * sizeof(HardRowInfo), pl.nrows * sizeof(HardRowInfo)}};
+  ci->nranges = 0;
+  ci->copy_bytes = 0;
+  for (const auto &r : rg) {
+    if (!r[1]) continue;
+    ci->off[ci->nranges] = r[0];
+    ci->len[ci->nranges] = r[1];
+    ci->copy_bytes += r[1];
+    ci->nranges++;
+  }
+  ci->hard_segments = pl.hard_segments;
+  return CPX_OK;
+}
+
+int stream_grow_tables(cpx_ctx *c, std::vector<HostQuery> &hq, bool p2_lookups)
+{
+  uint64_t R, P, Y;
+  stream_table_needs(c, hq, &R, &P, &Y, p2_lookups);
+  uint32_t cmax = 1;
+  for (const HostQuery &h : hq) cmax = std::max(cmax, h.c);
+  int rc = ensure_tables(c, R, P, Y);
+  if (!rc) rc = build_wheels(c, cmax);
+  return rc;
+}
+
+int stream_need_bytes(cpx_ctx *c, std::vector<HostQuery> &hq, uint64_t *bytes)
+{
+  int rc = stream_grow_tables(c, hq);
+  if (rc) return rc;
+  Plan pl;
+  if ((rc = build_plan(c, hq, TERM_ALL, pl, true))) return rc;
+  const Layout L = make_layout(needed_caps(pl));
+  *bytes = L.plan_bytes + L.out_bytes;
+  return CPX_OK;
+}
+
+BatchDev stream_batchdev(cpx_ctx *c, BatchSlot &S, ResultDev *d_results)
+{
+  BatchDev b = make_batchdev(c, S);
+  b.results = d_results;
+  return b;
 }

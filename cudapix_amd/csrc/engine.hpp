@@ -7,6 +7,7 @@
 #include <vector>
 #include <mutex>
 #include "cpx_internal.hpp"
+#include "cpx_stream.hpp"
 #include "../../include/cudapix.h"
 
 // Grow-only device buffer.
@@ -136,6 +137,41 @@ uint64_t slot_bytes(const cpx_ctx *c);
 uint64_t host_pi(const cpx_ctx *c, uint64_t v);   // v <= prime_limit
 uint64_t plan_bytes_estimate(const cpx_ctx *c, const HostQuery &hq);
 void free_tables(cpx_ctx *c);
+
+// ---- streaming cohorts (the server's continuous-batching launch)
+struct CohortInfo {
+  enum { OK = 0, TABLES = 1, CAPS = 2, TOO_BIG = 3 };
+  int why = OK;                  // why stream_plan returned CPX_EAGAIN / CPX_ENOMEM
+  uint32_t nq = 0, n_main = 0, n_fin = 0;
+  uint32_t nranges = 0;          // used prefixes of the plan arena (same offsets in the pinned image)
+  uint64_t off[12] = {}, len[12] = {};
+  uint64_t copy_bytes = 0;
+  uint64_t need_bytes = 0;       // plan + out bytes at exactly the capacities the plan needs
+  uint64_t hard_segments = 0;
+  uint64_t q_base = 0, tile_base = 0, fin_base = 0;   // its first query, tile, carry descriptor
+  Caps next{};                   // the arena cursor after its entries
+};
+bool tables_cover(const cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y);
+// p2_lookups: size the pi table for P2 by lookups of every eligible query.
+void stream_table_needs(const cpx_ctx *c, std::vector<HostQuery> &hq, uint64_t *R, uint64_t *P, uint64_t *Y,
+                        bool p2_lookups = false);
+// Plans hq as one streaming cohort into the arena S at the cursor (its
+// entries start there in every array, rounded up to CPX_STREAM_ALIGN; the
+// pinned image S.h_plan mirrors the plan region): ordinary and trivial as
+// tiles, stage table from the host, P2 by lookups where the resident table
+// reaches.  CPX_EAGAIN: the tables do not cover the cohort (why = TABLES) or
+// it does not fit behind the cursor and grow is false (CAPS); CPX_ENOMEM:
+// its state exceeds the budget (TOO_BIG).  With grow (only while no launch
+// uses the arena, cursor at zero) the arena is reallocated to hold mult
+// cohorts like this one.
+int stream_plan(cpx_ctx *c, std::vector<HostQuery> &hq, BatchSlot &S, const Caps &cursor, bool grow,
+                uint64_t budget, CohortInfo *ci, double mult = 1.0);
+int stream_grow_tables(cpx_ctx *c, std::vector<HostQuery> &hq, bool p2_lookups = false);
+// Device bytes of hq's state as one streaming cohort at exactly the
+// capacities its plan needs (grows the tables to cover it first).
+int stream_need_bytes(cpx_ctx *c, std::vector<HostQuery> &hq, uint64_t *bytes);
+
+BatchDev stream_batchdev(cpx_ctx *c, BatchSlot &S, ResultDev *d_results);
 
 // Below this x the answer is one pi-table lookup; at and above it sqrt(x) >= 128,
 // so the P2 range (which starts at sqrt(x) rounded down to a multiple of 64)

@@ -223,11 +223,26 @@ int cpx_get_tuning(cpx_ctx *ctx, uint32_t *threads, uint32_t *kwords, double *al
 int cpx_set_tuning(cpx_ctx *ctx, uint32_t threads, uint32_t kwords, double alpha_scale,
                    uint32_t wheel_c);
 
-/* ---- serving (continuous batching) --------------------------------------- */
+/* ---- serving (continuous batching) ---------------------------------------
+ * A server owns one executor context (the caller's tuning and tables are
+ * snapshotted at creation) and one streaming launch at a time: a persistent
+ * kernel that keeps taking work while the server publishes new cohorts of
+ * admitted requests into it.  Requests that arrive while others run join the
+ * running launch as soon as a cohort slot and the HBM budget allow; their
+ * tiles run beside the earlier cohorts' tail.  The reference has no serving
+ * path (src/main.cpp:23-88 runs one term per process). */
 typedef struct {
-  uint64_t hbm_budget;     /* bytes of per-request sieve state admitted at once (0: ctx budget) */
-  uint32_t max_batch;      /* largest admission group (default 256) */
-  double max_wait_ms;      /* how long a lone request waits for company (default 0.5) */
+  uint64_t hbm_budget;     /* device bytes for the cohort arena (0: 64 GiB): a cohort is admitted
+                              when its planned state fits the space left in the running launch;
+                              a request whose state alone exceeds the budget is rejected with
+                              CPX_ENOMEM */
+  uint32_t max_batch;      /* largest cohort (default 256) */
+  double max_wait_ms;      /* a pending request waits at most this long for the device backlog
+                              to drain (so that it joins a larger cohort) before it is
+                              published anyway (default 2) */
+  uint32_t slots;          /* cohorts in flight at once, 1..8 (default 4) */
+  cpx_u128 max_x;          /* build the resident tables for x up to this at creation (0: grow on
+                              demand, at a launch boundary) */
 } cpx_server_opts;
 
 typedef struct {
@@ -236,12 +251,18 @@ typedef struct {
   cpx_u128 pi;
   int status;              /* CPX_OK or a negative code */
   double t_arrive_ms, t_admit_ms, t_done_ms;   /* server clock */
+  uint64_t launch;         /* the streaming launch that served it (from 1) */
+  uint64_t cohort;         /* its cohort (from 1) */
+  double dev_first_ms, dev_last_ms;   /* device clock: its cohort's first main tile began, last ended */
 } cpx_result;
 
 typedef struct {
-  uint64_t submitted, completed, rejected, cancelled, batches;
-  uint64_t max_inflight_bytes;
+  uint64_t submitted, completed, rejected, cancelled, batches;   /* batches: cohorts published */
+  uint64_t max_inflight_bytes;   /* largest sum of the in-flight cohorts' arena needs */
   double p50_ms, p99_ms, mean_ms, qps;
+  uint64_t launches;       /* streaming launches */
+  uint64_t joined;         /* cohorts published into a launch that was already running another */
+  uint64_t arena_bytes;    /* device bytes reserved by the cohort slots */
 } cpx_server_stats;
 
 int cpx_server_create(cpx_ctx *ctx, const cpx_server_opts *o, cpx_server **out);
@@ -257,7 +278,10 @@ int cpx_server_destroy(cpx_server *s);
  * the calling thread); returns once the last request is queued. */
 int cpx_server_replay(cpx_server *s, const cpx_u128 *xs, const double *t_ms, size_t n,
                       uint64_t *tickets);
-/* Predicted device bytes of one request's sieve state (the admission price). */
+/* Device bytes of one request's state planned alone as a server cohort, at
+ * exactly the capacities its plan needs (the server admits a cohort by the
+ * same measure against a slot's share of its budget; 0 on error).  Grows the
+ * context's tables to cover x. */
 uint64_t cpx_request_bytes(cpx_ctx *ctx, cpx_u128 x);
 
 /* ---- host-side self checks (no GPU needed) -------------------------------- */

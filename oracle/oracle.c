@@ -13,6 +13,7 @@
  *   Ordinary   src/ordinary.cu:23-57,101-115  sum_{odd n<=y, lpf(n)>p_c} mu(n) phi(x/n, c)
  *   S2_trivial src/trivial.cu:31-62       C(pi(y)-pi(cbrt x), 2) + sum_{sqrt z<p<=cbrt x} [pi(y)-pi(x/p^2)]
  *   V (easy)   src/V.cu:22-124            sum_{x^1/4<p<=cbrt x} sum_{p<q<=min(y,x/p^2)} [pi(x/pq)-pi(p)+2]
+ *   V_a / V_b  src/V.cu:58-87 / :89-124   V split at p = sqrt(x/y) (orc_easy_part)
  *   S3 (hard)  src/sieve/S2_hard_device.cu:264-355 + S2_hard_host.cu:115-191 (DR split p <= x^1/4,
  *              src/Deleglise-Rivat/S3.cu:8): -sum_{p_c<p<=x^1/4} sum_{y/p<m<=y, lpf(m)>p} mu(m) phi(x/(mp), pi(p)-1)
  *   assembly   src/Deleglise-Rivat/deleglise-rivat.cpp:77
@@ -415,13 +416,20 @@ void orc_trivial(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, uint64_t ou
 /* V = sum_{x^(1/4) < p <= cbrt(x)} sum_{p < q <= min(y, x/p^2)} [pi(x/(pq)) - pi(p) + 2]
  * (src/V.cu:22-124, whose V_a/V_b split at p = sqrt(x/y) is the same set).
  * Every such leaf has x/(pq) < p^2, so phi(x/(pq), pi(p)-1) = pi(x/(pq)) - pi(p) + 2. */
-void orc_easy(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, uint64_t out[2])
+/* part: 0 = all of V; 1 = V_a, x^(1/4) < p <= sqrt(x/y) (q runs to y: src/V.cu:58-87);
+ * 2 = V_b, sqrt(x/y) < p <= cbrt(x) (q runs to x/p^2: src/V.cu:89-124).  The
+ * split point is isqrt(floor(x/y)) = floor(sqrt(x/y)). */
+void orc_easy_part(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, int part,
+                   uint64_t out[2])
 {
   u128 x = mk128(xhi, xlo);
   params_t P; make_params(&P, x, y, c);
   size_t np; uint32_t *pr = primes_upto(P.sqrtx, &np);   /* pi lookups up to sqrt x */
   uint64_t piy = count_le(pr, np, y);
   uint64_t j0 = count_le(pr, np, P.qrtx), j1 = count_le(pr, np, P.cbrtx);
+  uint64_t js = count_le(pr, np, P.sqrtz);
+  if (part == 1 && j1 > js) j1 = js;
+  if (part == 2 && j0 < js) j0 = js;
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -446,6 +454,11 @@ void orc_easy(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, u
   }
   put128((u128)s, out);
   free(pr);
+}
+
+void orc_easy(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, uint64_t out[2])
+{
+  orc_easy_part(xhi, xlo, y, c, threads, 0, out);
 }
 
 /* ------------------------------------------------------------ S3 (hard) */

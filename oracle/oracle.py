@@ -52,9 +52,11 @@ def lib():
         L.orc_ordinary.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
         L.orc_trivial.argtypes = [U64, U64, U64, ctypes.c_uint32, out2]
         L.orc_easy.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+        L.orc_easy_part.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, out2]
         L.orc_hard.argtypes = [U64, U64, U64, ctypes.c_uint32, out2]
         L.orc_pi_dr.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
-        for n in ("orc_p2", "orc_ordinary", "orc_trivial", "orc_easy", "orc_hard", "orc_pi_dr"):
+        for n in ("orc_p2", "orc_ordinary", "orc_trivial", "orc_easy", "orc_easy_part", "orc_hard",
+              "orc_pi_dr"):
             getattr(L, n).restype = None
         _lib = L
     return _lib
@@ -142,6 +144,16 @@ def S2_trivial(x, y, c=6):
 
 def V(x, y, c=6, threads=0):
     return _term(lib().orc_easy, *_split(x), y, c, threads)
+
+
+def V_a(x, y, c=6, threads=0):
+    """V over x^(1/4) < p <= sqrt(x/y) (src/V.cu:58-87)."""
+    return _term(lib().orc_easy_part, *_split(x), y, c, threads, 1)
+
+
+def V_b(x, y, c=6, threads=0):
+    """V over sqrt(x/y) < p <= cbrt(x) (src/V.cu:89-124)."""
+    return _term(lib().orc_easy_part, *_split(x), y, c, threads, 2)
 
 
 def S3(x, y, c=6):

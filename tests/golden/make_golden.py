@@ -8,8 +8,9 @@ written from the definitions (no reference code is imported or run):
   enumerated over ALL m (not only primes), with phi(n, b) read from cumulative
   sums of "no prime factor among the first b primes" indicator arrays, then
   classified by p into S1 (p > cbrt x), U (x/(p m) < p), V (x^(1/4) < p) and
-  S3 (p <= x^(1/4)); S0 from its definition with the same phi; P2 by counting
-  n <= x whose two prime factors (with multiplicity) both exceed y.
+  S3 (p <= x^(1/4)); V is also split at p = isqrt(x // y) into V_a (p at or
+  below) and V_b (above); S0 from its definition with the same phi; P2 by
+  counting n <= x whose two prime factors (with multiplicity) both exceed y.
 * known.json -- published values: pi(10^n) (OEIS A006880), pi(2^k)
   (OEIS A007053), Mertens M(10^n) (OEIS A084237).
 * queries_c3.json / queries_c4.json -- the batched and served workloads
@@ -104,7 +105,8 @@ def leaves_terms(x, alpha, c):
         if mu[n] != 0 and lpf[n] > pc:
             s0 += int(mu[n]) * phi_c(x // n)
     # special leaves
-    S1 = U = V = S3 = 0
+    S1 = U = V = S3 = Va = 0
+    sqz = iroot(z, 2)
     for bi, p in enumerate(primes):
         if p <= pc or p > y:
             continue
@@ -125,6 +127,8 @@ def leaves_terms(x, alpha, c):
             else:
                 assert mu[m] == -1          # m prime for every easy leaf
                 V += val
+                if p <= sqz:
+                    Va += val
     # P2 by brute count
     P2 = 0
     parr = np.array(primes, dtype=np.int64)
@@ -138,7 +142,7 @@ def leaves_terms(x, alpha, c):
     phi_xa = s0 + S1 + U + V + S3
     pi = phi_xa + a - 1 - P2
     return {"x": x, "alpha": alpha, "y": y, "c": c, "ordinary": s0, "trivial": S1 + U,
-            "easy": V, "hard": S3, "p2": P2, "pi_y": a, "pi": pi}
+            "easy": V, "easy_a": Va, "easy_b": V - Va, "hard": S3, "p2": P2, "pi_y": a, "pi": pi}
 
 
 def p2_brute(x, y):

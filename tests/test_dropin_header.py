@@ -1,7 +1,8 @@
 """CPU test: a reference-style C++ caller compiles against include/cudapix.hpp
 (the drop-in for P2.cuh, ordinary.cuh, trivial.cuh, S3.cuh, V.cuh,
 pitable.cuh, phi.cuh), links libcudapix_amd.so and runs its host-only self
-check.  Device calls are compiled but not executed here."""
+check.  Device calls are compiled but not executed here; the GPU test in
+test_gpu_boundary.py runs the same caller's device branch."""
 import os
 import shutil
 import subprocess
@@ -55,10 +56,9 @@ int main(int argc, char **argv) {
 """
 
 
-def test_reference_style_caller_builds_and_links(tmp_path):
+def build_caller(tmp_path):
+    """Compile CALLER against include/cudapix.hpp and link the library."""
     cxx = shutil.which("g++")
-    if not cxx:
-        pytest.skip("no g++")
     lib = os.path.join(ROOT, "cudapix_amd", "libcudapix_amd.so")
     assert os.path.exists(lib)
     src = tmp_path / "caller.cpp"
@@ -69,5 +69,12 @@ def test_reference_style_caller_builds_and_links(tmp_path):
            "-Wl,-rpath," + os.path.join(ROOT, "cudapix_amd"), "-L/opt/rocm/lib",
            "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True)
+    return exe
+
+
+def test_reference_style_caller_builds_and_links(tmp_path):
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    exe = build_caller(tmp_path)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     assert out.strip().splitlines()[-1] == "0"

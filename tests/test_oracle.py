@@ -28,6 +28,14 @@ def test_oracle_terms_match_brute_force_leaves(oracle, case):
         assert r[k] == case[k], k
 
 
+@pytest.mark.parametrize("case", load("terms.json"), ids=lambda t: "x%d-a%g-c%d" % (t["x"], t["alpha"], t["c"]))
+def test_oracle_v_split_matches_brute_force_leaves(oracle, case):
+    """V_a (x^(1/4) < p <= sqrt(x/y)) and V_b (sqrt(x/y) < p <= cbrt x), each
+    against the brute-force leaf classification on its own."""
+    assert oracle.V_a(case["x"], case["y"], case["c"]) == case["easy_a"]
+    assert oracle.V_b(case["x"], case["y"], case["c"]) == case["easy_b"]
+
+
 @pytest.mark.parametrize("case", load("p2_brute.json"), ids=lambda t: "x%d-y%d" % (t["x"], t["y"]))
 def test_oracle_p2_matches_semiprime_count(oracle, case):
     assert oracle.P2(case["x"], case["y"]) == case["p2"]
