@@ -96,7 +96,7 @@ EXPORTS = [
     "cpx_request_bytes", "cpx_selftest_arith", "cpx_selftest_wave", "cpx_probe_peaks",
     "cpx_get_tuning",
     "cpx_set_tuning",
-    "cpx_pi_batch_submit", "cpx_pi_batch_collect",
+    "cpx_pi_batch_submit", "cpx_pi_batch_collect", "cpx_pi_batch_noalloc",
     "cpx_get_device_primes32", "cpx_get_host_primes", "cpx_gen_device_bit_sieve",
 ]
 
@@ -167,6 +167,7 @@ def lib():
         "cpx_free": (ctypes.c_int, [vp, vp]),
         "cpx_pi_batch": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128),
                                         P(CpxBatchOpts), P(CpxBatchStats)]),
+        "cpx_pi_batch_noalloc": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128)]),
         "cpx_pi_batch_terms": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128),
                                               P(CpxTerms), P(CpxBatchOpts), P(CpxBatchStats)]),
         "cpx_autotune": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(ctypes.c_uint32),
@@ -454,6 +455,16 @@ class Context:
         if stats:
             return res, {f: getattr(st, f) for f, _ in CpxBatchStats._fields_}
         return res
+
+    def pi_batch_noalloc(self, xs):
+        """pi_batch with nothing allocated: the batch's state goes to the
+        library's static 128 MiB device array; the resident tables must
+        already cover it (CPX_EAGAIN), its state must fit (CPX_ENOMEM)."""
+        n = len(xs)
+        arr = (CpxU128 * max(n, 1))(*[u128(v) for v in xs])
+        out = (CpxU128 * max(n, 1))()
+        self._chk(self._L.cpx_pi_batch_noalloc(self.h, arr, n, out), "pi_batch_noalloc")
+        return [from_u128(out[i]) for i in range(n)]
 
     def pi_batch_submit(self, xs, use_graph=True, p2_table=0):
         """Plan and queue a batch without waiting; returns a ticket for

@@ -938,10 +938,10 @@ void free_slots(cpx_ctx *c)
   }
 }
 
-static BatchDev make_batchdev(cpx_ctx *c, const BatchSlot &S)
+// The launch record of a plan laid out as L, its plan arrays at P and its
+// output arrays at O (device addresses), over the context's tables.
+static BatchDev batchdev_at(const cpx_ctx *c, const Layout &L, char *P, char *O)
 {
-  const Layout &L = S.L;
-  char *P = (char *)S.d_plan, *O = (char *)S.d_out;
   BatchDev b{};
   b.pitab = c->pitab.p; b.pitab_n = c->pitab_nent;
   b.primes = c->primes.p; b.nprimes = c->nprimes; b.prinv = c->prinv.p;
@@ -970,6 +970,11 @@ static BatchDev make_batchdev(cpx_ctx *c, const BatchSlot &S)
   b.parts = (i128 *)(O + L.parts);
   b.results = (ResultDev *)(O + L.results);
   return b;
+}
+
+static BatchDev make_batchdev(cpx_ctx *c, const BatchSlot &S)
+{
+  return batchdev_at(c, S.L, (char *)S.d_plan, (char *)S.d_out);
 }
 
 // Writes the plan into a host image of the plan arena (layout L).
@@ -1295,6 +1300,65 @@ int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uin
   const int rc = submit_queries(c, hq, mask, p2mode, use_graph, &si);
   if (rc) return rc;
   return collect_queries(c, si, res, st);
+}
+
+// ------------------------------------------- batches in a caller's region
+
+int run_queries_at(cpx_ctx *c, std::vector<HostQuery> &hq, void *d_region, uint64_t region_bytes,
+                   std::vector<ResultDev> &res)
+{
+  if (hq.empty()) { res.clear(); return CPX_OK; }
+  // P2 by lookups only where the resident pi table already reaches
+  choose_p2tab(c, hq, TERM_ALL, 1);
+  uint64_t R = 0, P = 0, Y = 0;
+  uint32_t cmax = 1;
+  for (const HostQuery &h : hq) {
+    plan_needs(c, h, TERM_ALL, c->shape.seg_bits(), &R, &P, &Y);
+    cmax = std::max(cmax, h.c);
+  }
+  if (!tables_cover(c, R, P, Y) || !c->wheel.p || cmax > c->wheel_c)
+    return ctx_fail(c, CPX_EAGAIN, "the resident tables do not cover this batch (an allocating call at this size "
+                                   "builds them)");
+  Plan pl;
+  int rc = build_plan(c, hq, TERM_ALL, pl);
+  if (rc) return rc;
+  Caps k = needed_caps(pl);
+  k.q = std::max<uint64_t>(k.q, 64);
+  k.groups = std::max<uint64_t>(k.groups, 16);
+  const Layout L = make_layout(k);
+  const uint64_t plan_al = (L.plan_bytes + 255) & ~255ull;
+  if (plan_al + L.out_bytes > region_bytes)
+    return ctx_fail(c, CPX_ENOMEM, "batch state exceeds the caller's device region");
+  char *P0 = (char *)d_region, *O0 = P0 + plan_al;
+  // the plan from a host image, used prefix of each array (stream order: the
+  // previous call's kernels on this stream have read the region before)
+  std::vector<char> H(L.plan_bytes);
+  pack_plan(L, pl, H.data());
+  auto up = [&](size_t off, size_t bytes) -> hipError_t {
+    if (!bytes) return hipSuccess;
+    return hipMemcpyAsync(P0 + off, H.data() + off, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(up(L.counters, 64));
+  HIPCHK(up(L.queries, pl.q.size() * sizeof(QueryDev)));
+  HIPCHK(up(L.tiles, pl.tiles.size() * sizeof(TileDev)));
+  HIPCHK(up(L.groups, pl.groups.size() * sizeof(GroupDev)));
+  HIPCHK(up(L.ord, pl.nord * sizeof(BlockDesc)));
+  HIPCHK(up(L.triv, pl.ntriv * sizeof(BlockDesc)));
+  HIPCHK(up(L.easy, pl.neasy * sizeof(BlockDesc)));
+  HIPCHK(up(L.p2d, pl.np2d * sizeof(BlockDesc)));
+  HIPCHK(up(L.fin, pl.nfin * sizeof(BlockDesc)));
+  HIPCHK(up(L.rowinfo, pl.nrows * sizeof(HardRowInfo)));
+  const BatchDev b = batchdev_at(c, L, P0, O0);
+  const uint32_t nres_words = (uint32_t)(k.q * sizeof(ResultDev) / 4);
+  for (int part = 0; part < 3; part++)
+    if ((rc = enqueue_part(c, b, part, nres_words))) return rc;
+  res.resize(hq.size());
+  HIPCHK(hipMemcpyAsync(res.data(), O0 + L.results, hq.size() * sizeof(ResultDev), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (const ResultDev &r : res)
+    if (r.status != 1) return ctx_fail(c, CPX_EDEVICE, "a query produced no result");
+  return CPX_OK;
 }
 
 // ------------------------------------------------------ streaming cohorts

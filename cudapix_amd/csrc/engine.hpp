@@ -131,6 +131,13 @@ int submit_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, 
 int collect_queries(cpx_ctx *c, uint32_t slot, std::vector<ResultDev> &res, cpx_batch_stats *st);
 int run_queries(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, uint32_t p2mode,
                 std::vector<ResultDev> &res, cpx_batch_stats *st, bool use_graph);
+// A batch with its plan and output arrays in a device region the caller
+// provides (region_bytes long), nothing allocated on the device or in pinned
+// host memory: the resident tables must already cover it (CPX_EAGAIN
+// otherwise), and its state must fit the region (CPX_ENOMEM otherwise).
+// Runs on the context's stream and waits for it.
+int run_queries_at(cpx_ctx *c, std::vector<HostQuery> &hq, void *d_region, uint64_t region_bytes,
+                   std::vector<ResultDev> &res);
 // Frees every slot's buffers, graphs and events (context teardown).
 void free_slots(cpx_ctx *c);
 uint64_t slot_bytes(const cpx_ctx *c);

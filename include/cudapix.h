@@ -193,6 +193,20 @@ int cpx_pi_batch(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
 /* Same, also returning each query's term breakdown (terms may be NULL). */
 int cpx_pi_batch_terms(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out,
                        cpx_terms *terms, const cpx_batch_opts *opts, cpx_batch_stats *stats);
+/* cpx_pi_batch without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The batch's plan and
+ * output arrays go to a static 128 MiB device array of the library (each GPU
+ * the library runs on holds one until the process ends) instead of the
+ * context's arena; nothing is allocated on the device or in pinned host
+ * memory (host planning uses the process heap).  The context's resident
+ * tables must already cover the batch -- an allocating call (cpx_pi_batch)
+ * at the largest x builds them -- else CPX_EAGAIN; a batch whose state
+ * exceeds 128 MiB (about one x near 8e16, or many smaller ones) gives
+ * CPX_ENOMEM.  Waits for its results.  Calls on one GPU share its array and
+ * are serialized by the library; the array keeps the last call's contents
+ * (every call clears what must start at zero). */
+int cpx_pi_batch_noalloc(cpx_ctx *ctx, const cpx_u128 *xs, size_t n, cpx_u128 *out);
 /* Pipelined batches (no reference counterpart: the reference solves one x per
  * run).  cpx_pi_batch_submit plans the batch on the calling thread, queues its
  * upload and launches and returns a ticket without waiting, so the next batch
