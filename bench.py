@@ -444,14 +444,33 @@ def main():
             t0 = pairs[0][0]
             sx = [p[1] for p in pairs]
             st_ms = [p[0] - t0 for p in pairs]
-            srv = C.Server(ctx)
-            srv.replay(sx, st_ms)
+            # every x of the stream solved alone (one batch), to check each served value
+            uniq = sorted(set(sx))
+            alone = dict(zip(uniq, ctx.pi_batch(uniq)))
+            srv = C.Server(ctx, max_x=max(sx))
+            tickets = srv.replay(sx, st_ms)
             srv.drain()
             res = srv.poll(len(sx) + 16)
             stt = srv.stats()
             srv.close()
             got = {r["ticket"]: r for r in res}
             ok = all(r["status"] == 0 for r in res) and len(res) == len(sx)
+            matches = sum(1 for tk, x in zip(tickets, sx)
+                          if tk in got and got[tk]["status"] == 0 and got[tk]["pi"] == alone[x])
+            if matches != len(sx):
+                raise SystemExit("served lambda=%d: %d of %d results differ from the same x solved alone"
+                                 % (lam, len(sx) - matches, len(sx)))
+            # the same x list one query at a time (warm tables, no batching)
+            pw = None
+            if not args.no_plain:
+                with C.Context(device=0, use_graphs=False, seg_threads=shape[0],
+                               seg_kwords=shape[1]) as wc:
+                    wc.pi(max(sx))
+                    t0p = time.time()
+                    for x in sx:
+                        if wc.pi(x) != alone[x]:
+                            raise SystemExit("plain solve of %d differs" % x)
+                    pw = len(sx) / (time.time() - t0p)
             waits = sorted(r["t_admit_ms"] - r["t_arrive_ms"] for r in res)
             qw = (lambda f: round(waits[min(len(waits) - 1, int(f * len(waits)))], 3)) if waits else \
                 (lambda f: None)
@@ -461,9 +480,12 @@ def main():
                                 "queue_p50_ms": qw(0.5), "queue_p99_ms": qw(0.99),
                                 "submitted": stt["submitted"], "completed": stt["completed"],
                                 "rejected": stt["rejected"], "batches": stt["batches"],
-                                "max_inflight_bytes": stt["max_inflight_bytes"], "all_ok": ok}
+                                "max_inflight_bytes": stt["max_inflight_bytes"],
+                                "launches": stt["launches"], "joined": stt["joined"],
+                                "all_ok": ok, "matches_alone": matches,
+                                "plain_warm_qps_same_x": round(pw, 2) if pw else None,
+                                "speedup_vs_plain_warm": round(stt["qps"] / pw, 2) if pw else None}
             log("served lambda=%d: %s" % (lam, served[str(lam)]))
-        # each served result equals the same x solved alone
         out["served"] = served
 
     # ---- roofline of the persistent segment kernel (the LDS bound of the

@@ -89,7 +89,9 @@ EXPORTS = [
     "cpx_pitable_create", "cpx_pitable_destroy", "cpx_pitable_current",
     "cpx_pitable_next_up", "cpx_pitable_next_down", "cpx_pitable_info",
     "cpx_phi_create", "cpx_phi_destroy", "cpx_phi_eval", "cpx_phi_range",
-    "cpx_gen_d_lpf", "cpx_gen_d_mu", "cpx_free",
+    "cpx_gen_d_lpf", "cpx_gen_d_mu", "cpx_free", "cpx_gen_d_lpf64", "cpx_copy_to_host",
+    "cpx_pitable_set_range", "cpx_pitable_set_pi_base", "cpx_pitable_max_range",
+    "cpx_phi_create_pmax", "cpx_phi_lpf", "cpx_phi_prime_list", "cpx_phi_mod_cache",
     "cpx_pi_batch", "cpx_pi_batch_terms", "cpx_autotune",
     "cpx_server_create", "cpx_server_submit", "cpx_server_cancel", "cpx_server_poll",
     "cpx_server_drain", "cpx_server_stats_get", "cpx_server_destroy", "cpx_server_replay",
@@ -163,6 +165,15 @@ def lib():
         "cpx_phi_eval": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, P(U64)]),
         "cpx_phi_range": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, P(vp)]),
         "cpx_gen_d_lpf": (ctypes.c_int, [vp, U64, U64, P(vp)]),
+        "cpx_gen_d_lpf64": (ctypes.c_int, [vp, U64, U64, P(vp)]),
+        "cpx_copy_to_host": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
+        "cpx_pitable_set_range": (ctypes.c_int, [vp, U64]),
+        "cpx_pitable_set_pi_base": (ctypes.c_int, [vp, U64]),
+        "cpx_pitable_max_range": (ctypes.c_int, [vp, P(U64)]),
+        "cpx_phi_create_pmax": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, P(vp)]),
+        "cpx_phi_lpf": (ctypes.c_int, [vp, P(vp), P(U64)]),
+        "cpx_phi_prime_list": (ctypes.c_int, [vp, P(vp), P(vp), P(ctypes.c_uint32)]),
+        "cpx_phi_mod_cache": (ctypes.c_int, [vp, ctypes.c_uint32, P(U64)]),
         "cpx_gen_d_mu": (ctypes.c_int, [vp, U64, U64, P(vp)]),
         "cpx_free": (ctypes.c_int, [vp, vp]),
         "cpx_pi_batch": (ctypes.c_int, [vp, P(CpxU128), ctypes.c_size_t, P(CpxU128),
@@ -411,6 +422,15 @@ class Context:
         finally:
             self._L.cpx_free(self.h, d)
 
+    def gen_lpf64(self, bottom, top):
+        """gen_d_lpf<uint64_t>: 64-bit entries, windows anywhere below 2^64 - 2^34."""
+        d = ctypes.c_void_p()
+        self._chk(self._L.cpx_gen_d_lpf64(self.h, bottom, top, ctypes.byref(d)), "gen_d_lpf64")
+        try:
+            return self._d2h(d, (top - bottom) // 2, U64)
+        finally:
+            self._L.cpx_free(self.h, d)
+
     def gen_mu(self, bottom, top):
         d = ctypes.c_void_p()
         self._chk(self._L.cpx_gen_d_mu(self.h, bottom, top, ctypes.byref(d)), "gen_d_mu")
@@ -581,6 +601,17 @@ class PiTable:
     def getNextDown(self):
         return self._get(self.ctx._L.cpx_pitable_next_down)
 
+    def set_pi_base(self, pi_base):
+        self.ctx._chk(self.ctx._L.cpx_pitable_set_pi_base(self.h, pi_base), "PiTable.set_pi_base")
+
+    def reallocate(self, range_):
+        self.ctx._chk(self.ctx._L.cpx_pitable_set_range(self.h, range_), "PiTable.reallocate")
+
+    def getMaxRange(self):
+        r = U64()
+        self.ctx._chk(self.ctx._L.cpx_pitable_max_range(self.ctx.h, ctypes.byref(r)), "PiTable.getMaxRange")
+        return r.value
+
     def close(self):
         if self.h:
             self.ctx._L.cpx_pitable_destroy(self.h)
@@ -590,11 +621,34 @@ class PiTable:
 class Phi:
     """phi(x, a) for x < max from a device LPF table (class Phi, include/phi.cuh:17-36)."""
 
-    def __init__(self, ctx, max_):
+    def __init__(self, ctx, max_, p_max=0):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        ctx._chk(ctx._L.cpx_phi_create(ctx.h, max_, ctypes.byref(h)), "Phi")
+        ctx._chk(ctx._L.cpx_phi_create_pmax(ctx.h, max_, p_max, ctypes.byref(h)), "Phi")
         self.h = h
+
+    def get_d_lpf(self):
+        """(device pointer, entries) of the object's odd-only LPF array."""
+        d, n = ctypes.c_void_p(), U64()
+        self.ctx._chk(self.ctx._L.cpx_phi_lpf(self.h, ctypes.byref(d), ctypes.byref(n)), "Phi.get_d_lpf")
+        return d, n.value
+
+    def lpf(self):
+        d, n = self.get_d_lpf()
+        return self.ctx._d2h(d, n, ctypes.c_uint32)
+
+    def prime_list(self):
+        """(host list, device list copied back) of get_h_primeList / get_d_primeList."""
+        d, h, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_uint32()
+        self.ctx._chk(self.ctx._L.cpx_phi_prime_list(self.h, ctypes.byref(d), ctypes.byref(h),
+                                                     ctypes.byref(n)), "Phi.primeList")
+        hv = list(ctypes.cast(h, ctypes.POINTER(ctypes.c_uint32))[:n.value]) if n.value else []
+        return hv, self.ctx._d2h(d, n.value, ctypes.c_uint32)
+
+    def makeModCache(self, pi_max):
+        out = (U64 * max(pi_max - 1, 1))()
+        self.ctx._chk(self.ctx._L.cpx_phi_mod_cache(self.h, pi_max, out), "Phi.makeModCache")
+        return list(out[:max(pi_max - 1, 0)])
 
     def phi(self, x, a):
         o = U64()

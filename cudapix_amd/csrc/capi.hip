@@ -380,30 +380,52 @@ int cpx_free(cpx_ctx *c, void *p)
   return CPX_OK;
 }
 
+int cpx_copy_to_host(cpx_ctx *c, void *dst, const void *d_src, size_t bytes)
+{
+  CTX_GUARD(c);
+  if (bytes && (!dst || !d_src)) return CPX_EINVAL;
+  if (!bytes) return CPX_OK;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CPX_OK;
+}
+
 struct cpx_pitable {
   cpx_ctx *ctx;
   uint64_t base, bottom, range, pi_base;
   uint32_t *d = nullptr;
+  bool pi_set = false;          // pi_base given by the caller (set_pi_base): carried, not counted
 };
+
+// Primes in [lo, hi) from the resident pi table (which reaches hi).
+static int pitable_count(cpx_ctx *c, uint64_t lo, uint64_t hi, uint64_t *out)
+{
+  *out = 0;
+  if (hi <= lo) return CPX_OK;
+  unsigned long long *d;
+  HIPCHK(hipMalloc(&d, 8));
+  hipError_t e = launch_count_range(c->pitab.p, lo, hi, d, c->stream);
+  unsigned long long h = 0;
+  if (!e) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e) return hip_fail(c, e, "pi table count");
+  *out = h;
+  return CPX_OK;
+}
 
 static int pitable_fill(cpx_pitable *t)
 {
   cpx_ctx *c = t->ctx;
   int rc = ensure_tables(c, t->base + t->range + 2, 64, 64);
   if (rc) return rc;
+  // the table holds pi(n) - pi(base) (src/pitable.cu:196-218 writes the
+  // window's own prime index); pi_base is reported beside it, counted unless
+  // the caller set it
   uint64_t pb = 0;
-  if (t->base >= 2) {
-    unsigned long long *d;
-    HIPCHK(hipMalloc(&d, 8));
-    hipError_t e = launch_count_range(c->pitab.p, 0, t->base, d, c->stream);
-    unsigned long long h = 0;
-    if (!e) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream);
-    if (!e) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d);
-    if (e) return hip_fail(c, e, "pi_base");
-    pb = h;
-  }
-  t->pi_base = pb;
+  if (t->base >= 2 && (rc = pitable_count(c, 0, t->base, &pb))) return rc;
+  if (!t->pi_set) t->pi_base = pb;
   HIPCHK(launch_expand_pitab_u32(c->pitab.p, t->d, t->base, t->range / 2 + 2, pb, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CPX_OK;
@@ -441,6 +463,14 @@ int cpx_pitable_current(cpx_pitable *t, uint32_t **d)
 int cpx_pitable_next_up(cpx_pitable *t, uint32_t **d)
 {
   if (!t || !d) return CPX_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+  if (t->pi_set) {                 // a caller's pi_base moves up by the window's primes
+    int rc = ensure_tables(t->ctx, t->base + t->range + 2, 64, 64);
+    uint64_t n = 0;
+    if (!rc) rc = pitable_count(t->ctx, t->base, t->base + t->range, &n);
+    if (rc) return rc;
+    t->pi_base += n;
+  }
   t->base += t->range;
   return cpx_pitable_current(t, d);
 }
@@ -448,10 +478,51 @@ int cpx_pitable_next_up(cpx_pitable *t, uint32_t **d)
 int cpx_pitable_next_down(cpx_pitable *t, uint32_t **d)
 {
   if (!t || !d) return CPX_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+  const uint64_t b0 = t->base;
   if (t->base >= t->bottom + t->range) t->base -= t->range;
   else { t->range = t->base - t->bottom; t->base = t->bottom; }
   if (t->range == 0) return CPX_ERANGE;
+  if (t->pi_set) {                 // ... and down by the new window's
+    uint64_t n = 0;
+    const int rc = pitable_count(t->ctx, t->base, b0, &n);
+    if (rc) return rc;
+    t->pi_base -= n;
+  }
   return cpx_pitable_current(t, d);
+}
+
+int cpx_pitable_set_range(cpx_pitable *t, uint64_t range)
+{
+  if (!t || range == 0) return CPX_EINVAL;
+  cpx_ctx *c = t->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, (range / 2 + 2) * 4));
+  if (t->d) (void)hipFree(t->d);
+  t->d = d;
+  t->range = range;
+  return CPX_OK;
+}
+
+int cpx_pitable_set_pi_base(cpx_pitable *t, uint64_t pi_base)
+{
+  if (!t) return CPX_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+  t->pi_base = pi_base;
+  t->pi_set = true;
+  return CPX_OK;
+}
+
+int cpx_pitable_max_range(cpx_ctx *c, uint64_t *range)
+{
+  CTX_GUARD(c);
+  if (!range) return CPX_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  *range = (uint64_t)(fr / 2.5) & ~1ull;      // a table of range/2 words then takes 80 % of the free bytes
+  return CPX_OK;
 }
 
 int cpx_pitable_info(cpx_pitable *t, uint64_t *base, uint64_t *range, uint64_t *pi_base)
@@ -468,18 +539,22 @@ struct cpx_phi {
   uint32_t max;
   uint32_t *d_lpf = nullptr;     // odd k = 2i + 1 < max
   uint64_t n = 0;
+  uint32_t p_max = 0;            // the prime list ends at this bound
+  uint32_t *d_plist = nullptr;   // the prime list (made on first request)
+  std::vector<uint32_t> h_plist;
 };
 
-int cpx_phi_create(cpx_ctx *c, uint32_t max, cpx_phi **out)
+int cpx_phi_create_pmax(cpx_ctx *c, uint32_t max, uint32_t p_max, cpx_phi **out)
 {
   CTX_GUARD(c);
   if (!out || max < 3) return CPX_EINVAL;
   *out = nullptr;
   HIPCHK(hipSetDevice(c->device));
-  uint64_t r = cpx_isqrt128((u128)max + 16384) + 2;
+  if (p_max == 0) p_max = (uint32_t)cpx_isqrt128((u128)max);
+  uint64_t r = std::max<uint64_t>(cpx_isqrt128((u128)max + 16384) + 2, (uint64_t)p_max + 2);
   int rc = ensure_tables(c, r, r, 64);
   if (rc) return rc;
-  cpx_phi *p = new cpx_phi{c, max, nullptr, 0};
+  cpx_phi *p = new cpx_phi{c, max, nullptr, 0, p_max, nullptr, {}};
   p->n = ((uint64_t)max + 1) / 2;
   uint64_t nalloc = (p->n + 4095) / 4096 * 4096;
   hipError_t e = hipMalloc(&p->d_lpf, nalloc * 4);
@@ -490,11 +565,75 @@ int cpx_phi_create(cpx_ctx *c, uint32_t max, cpx_phi **out)
   return CPX_OK;
 }
 
+int cpx_phi_create(cpx_ctx *c, uint32_t max, cpx_phi **out)
+{
+  return cpx_phi_create_pmax(c, max, 0, out);
+}
+
 int cpx_phi_destroy(cpx_phi *p)
 {
   if (!p) return CPX_EINVAL;
   if (p->d_lpf) (void)hipFree(p->d_lpf);
+  if (p->d_plist) (void)hipFree(p->d_plist);
   delete p;
+  return CPX_OK;
+}
+
+int cpx_phi_lpf(cpx_phi *p, uint32_t **d_lpf, uint64_t *len)
+{
+  if (!p || !d_lpf) return CPX_EINVAL;
+  *d_lpf = p->d_lpf;
+  if (len) *len = p->n;
+  return CPX_OK;
+}
+
+// The sieving-prime list of the reference's Phi: src/phi.cu:44-45 indexes it
+// as the continuation of its own table of the first twelve primes (a > 12
+// reads h_primeList[a - 12]), so it runs from the 13th prime, 41, to p_max.
+int cpx_phi_prime_list(cpx_phi *p, uint32_t **d_list, const uint32_t **h_list, uint32_t *len)
+{
+  if (!p) return CPX_EINVAL;
+  cpx_ctx *c = p->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!p->d_plist) {
+    std::vector<uint32_t> v;
+    for (uint64_t i = 0; i < c->nprimes && c->h_primes[i] <= p->p_max; i++)
+      if (c->h_primes[i] >= 41) v.push_back(c->h_primes[i]);
+    uint32_t *d = nullptr;
+    HIPCHK(hipMalloc(&d, std::max<size_t>(v.size(), 1) * 4));
+    hipError_t e = v.empty() ? hipSuccess
+                             : hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) { (void)hipFree(d); return hip_fail(c, e, "phi prime list"); }
+    p->d_plist = d;
+    p->h_plist.swap(v);
+  }
+  if (d_list) *d_list = p->d_plist;
+  if (h_list) *h_list = p->h_plist.data();
+  if (len) *len = (uint32_t)p->h_plist.size();
+  return CPX_OK;
+}
+
+int cpx_phi_mod_cache(cpx_phi *p, uint32_t pi_max, uint64_t *out)
+{
+  if (!p || (pi_max > 1 && !out)) return CPX_EINVAL;
+  cpx_ctx *c = p->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (pi_max > 16) return ctx_fail(c, CPX_ERANGE, "pi_max above 16 (the primorial leaves 64 bits)");
+  static const uint32_t sp[16] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53};
+  uint64_t w = 1, tot = 1;
+  for (uint32_t k = 1; k < pi_max; k++) {
+    w *= sp[k - 1];
+    tot *= sp[k - 1] - 1;
+    // phi(p_1...p_k, k) = prod (p_i - 1); counted by the table where it reaches
+    if (w < p->max) {
+      uint64_t v = 0;
+      const int rc = cpx_phi_eval(p, (uint32_t)w, k, &v);
+      if (rc) return rc;
+      if (v != tot) return ctx_fail(c, CPX_EDEVICE, "phi of a primorial disagrees with its totient");
+    }
+    out[k - 1] = tot;
+  }
   return CPX_OK;
 }
 
@@ -560,10 +699,14 @@ int cpx_gen_d_lpf(cpx_ctx *c, uint64_t bottom, uint64_t top, uint32_t **d_out)
   return CPX_OK;
 }
 
+// Windows of 64-bit n end below this (a sieve segment past top and the
+// sieving-prime limit sqrt(top) stay inside 64 and 32 bits).
+static const uint64_t LPF64_TOP_MAX = ~0ull - (1ull << 34);
+
 int cpx_gen_d_mu(cpx_ctx *c, uint64_t bottom, uint64_t top, int8_t **d_out)
 {
   CTX_GUARD(c);
-  if (!d_out || (bottom & 1) || top <= bottom || top >= (1ull << 32)) return CPX_EINVAL;
+  if (!d_out || (bottom & 1) || top <= bottom || top > LPF64_TOP_MAX) return CPX_EINVAL;
   *d_out = nullptr;
   uint64_t r = cpx_isqrt128((u128)top + 16384) + 2;
   int rc = ensure_tables(c, r, r, 64);
@@ -571,9 +714,31 @@ int cpx_gen_d_mu(cpx_ctx *c, uint64_t bottom, uint64_t top, int8_t **d_out)
   uint64_t n = (top - bottom) / 2, nalloc = (n + 4095) / 4096 * 4096;
   int8_t *d = nullptr;
   HIPCHK(hipMalloc(&d, nalloc));
-  hipError_t e = launch_mu_lpf_range(nullptr, d, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream);
+  // 32-bit cofactors while the window's last segment stays below 2^32
+  hipError_t e = bottom + 2 * nalloc < (1ull << 32)
+                     ? launch_mu_lpf_range(nullptr, d, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream)
+                     : launch_mu_lpf64_range(nullptr, d, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes,
+                                             c->stream);
   if (!e) e = hipStreamSynchronize(c->stream);
   if (e) { (void)hipFree(d); return hip_fail(c, e, "gen_d_mu"); }
+  *d_out = d;
+  return CPX_OK;
+}
+
+int cpx_gen_d_lpf64(cpx_ctx *c, uint64_t bottom, uint64_t top, uint64_t **d_out)
+{
+  CTX_GUARD(c);
+  if (!d_out || (bottom & 1) || top <= bottom || top > LPF64_TOP_MAX) return CPX_EINVAL;
+  *d_out = nullptr;
+  uint64_t r = cpx_isqrt128((u128)top + 16384) + 2;
+  int rc = ensure_tables(c, r, r, 64);
+  if (rc) return rc;
+  uint64_t n = (top - bottom) / 2, nalloc = (n + 4095) / 4096 * 4096;
+  uint64_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, nalloc * 8));
+  hipError_t e = launch_mu_lpf64_range(d, nullptr, bottom / 2, nalloc, c->primes.p, (uint32_t)c->nprimes, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (e) { (void)hipFree(d); return hip_fail(c, e, "gen_d_lpf64"); }
   *d_out = d;
   return CPX_OK;
 }

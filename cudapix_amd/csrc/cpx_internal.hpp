@@ -230,6 +230,8 @@ size_t seg_lds_bytes(SegShape shape);
 bool seg_shape_supported(SegShape shape);
 hipError_t launch_scan_entries(PiEntry *tab, uint64_t nent, uint32_t *tot, uint64_t *base,
                                hipStream_t s);
+hipError_t launch_mu_lpf64_range(uint64_t *lpf, int8_t *mu, uint64_t ibase, uint64_t nodd,
+                                 const uint32_t *primes, uint32_t nsp, hipStream_t s);
 hipError_t launch_mu_lpf_split(uint32_t *lpf, int8_t *mu, uint64_t nodd, const uint32_t *primes,
                                uint32_t nsp, hipStream_t s);
 hipError_t launch_phi_count_lpf(const uint32_t *lpf, uint64_t i0, uint64_t i1, uint32_t pa,

@@ -288,6 +288,65 @@ hipError_t launch_mu_lpf_split(uint32_t *lpf, int8_t *mu, uint64_t nodd, const u
   return hipGetLastError();
 }
 
+// The same sieve for odd m of 64 bits (a window anywhere below 2^64 - 2^34):
+// the cofactor in 64 bits, exact division by p^-1 mod 2^64 (Newton from p,
+// five steps for 3 -> 96 correct bits), the divisibility test x*inv <=
+// (2^64-1)/p, and the lpf of a prime m above 2^32 kept whole.  Sieving
+// primes reach sqrt(top) < 2^32.  (The 64-bit instances of
+// src/sieve/lpf_mu.cu:57-87,153-197.)
+__global__ __launch_bounds__(256) void k_mlpf64(uint64_t *lpf_out, int8_t *mu_out, uint64_t ibase, uint64_t nodd,
+                                                const uint32_t *__restrict__ sp, uint32_t nsp)
+{
+  __shared__ uint64_t r[MLPF_SEG];
+  __shared__ uint32_t lp[MLPF_SEG];
+  __shared__ uint8_t fl[MLPF_SEG];                 // parity (bit 0), squareful (bit 1)
+  const uint32_t t = threadIdx.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * MLPF_SEG;
+  const uint64_t m0 = 2 * (ibase + i0);
+  for (uint32_t i = t; i < MLPF_SEG; i += blockDim.x) {
+    r[i] = m0 + 2 * i + 1;
+    lp[i] = 0;
+    fl[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t mhi = m0 + 2 * MLPF_SEG;
+  for (uint32_t k = 1; k < nsp; k++) {
+    const uint32_t p = sp[k];
+    if ((uint64_t)p * p >= mhi) break;
+    uint64_t inv = p;
+    for (int it = 0; it < 5; it++) inv *= 2u - p * inv;
+    const uint64_t lim = ~0ull / p;
+    const uint64_t o = first_odd_multiple_bit(m0, p, p);
+    for (uint64_t j = o + t * (uint64_t)p; j < MLPF_SEG; j += (uint64_t)blockDim.x * p) {
+      const uint64_t q = r[j] * inv;
+      r[j] = q;
+      if (!lp[j]) lp[j] = p;
+      uint32_t f = fl[j] ^ 1u;
+      if (q * inv <= lim) f |= 2u;
+      fl[j] = (uint8_t)f;
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = t; i < MLPF_SEG; i += blockDim.x) {
+    const uint64_t idx = i0 + i;
+    if (idx >= nodd) break;
+    uint32_t par = fl[i] & 1u;
+    const bool sq = (fl[i] >> 1) & 1u;
+    uint64_t lpf = lp[i];
+    if (r[i] > 1) { par ^= 1u; if (!lpf) lpf = r[i]; }
+    if (lpf_out) lpf_out[idx] = lpf ? lpf : ~0ull;                // m = 1
+    if (mu_out) mu_out[idx] = (int8_t)(sq ? 0 : (par ? -1 : 1));
+  }
+}
+
+hipError_t launch_mu_lpf64_range(uint64_t *lpf, int8_t *mu, uint64_t ibase, uint64_t nodd,
+                                 const uint32_t *primes, uint32_t nsp, hipStream_t s)
+{
+  uint32_t g = (uint32_t)((nodd + MLPF_SEG - 1) / MLPF_SEG);
+  hipLaunchKernelGGL(k_mlpf64, dim3(g), dim3(256), 0, s, lpf, mu, ibase, nodd, primes, nsp);
+  return hipGetLastError();
+}
+
 // lpf / mu of the odd n = 2(ibase + i) + 1, i < nodd (gen_d_lpf / gen_d_mu with an
 // even bottom = 2 * ibase).
 hipError_t launch_mu_lpf_range(uint32_t *lpf, int8_t *mu, uint64_t ibase, uint64_t nodd,

@@ -153,18 +153,49 @@ int cpx_pitable_current(cpx_pitable *t, uint32_t **d_table);   /* getCurrent */
 int cpx_pitable_next_up(cpx_pitable *t, uint32_t **d_table);   /* getNextUp */
 int cpx_pitable_next_down(cpx_pitable *t, uint32_t **d_table); /* getNextDown */
 int cpx_pitable_info(cpx_pitable *t, uint64_t *base, uint64_t *range, uint64_t *pi_base);
+/* PiTable::reallocate after set_range  include/pitable.cuh:41 (a table for
+ * the new range; filled on the next current/next_up/next_down). */
+int cpx_pitable_set_range(cpx_pitable *t, uint64_t range);
+/* PiTable::set_pi_base  include/pitable.cuh:34: pi(base) given by the caller
+ * (not counted); next_up / next_down carry it by the windows' prime counts. */
+int cpx_pitable_set_pi_base(cpx_pitable *t, uint64_t pi_base);
+/* PiTable::getMaxRange  include/pitable.cuh:38: the range whose table takes
+ * 80 % of the device's free bytes (free / 2.5, as src/pitable.cu:45-51). */
+int cpx_pitable_max_range(cpx_ctx *ctx, uint64_t *range);
 /* class Phi  include/phi.cuh:17-36: phi(x, a) for x < max from an LPF table. */
 int cpx_phi_create(cpx_ctx *ctx, uint32_t max, cpx_phi **out);
+/* Phi(max, p_max)  include/phi.cuh:28: the prime list ends at p_max (0: sqrt(max)). */
+int cpx_phi_create_pmax(cpx_ctx *ctx, uint32_t max, uint32_t p_max, cpx_phi **out);
+/* Phi::get_d_lpf  include/phi.cuh:23: the object's odd-only LPF array (index
+ * i <-> k = 2i + 1 < max, lpf(1) = UINT32_MAX; len = its entries). */
+int cpx_phi_lpf(cpx_phi *p, uint32_t **d_lpf, uint64_t *len);
+/* Phi::get_d_primeList / get_h_primeList / get_primeListLength
+ * include/phi.cuh:24-26: the primes from 41 (the 13th prime: the reference's
+ * phi() takes the first twelve from a table of its own, src/phi.cu:11,44-45)
+ * to p_max, on the device and on the host (both owned by the object). */
+int cpx_phi_prime_list(cpx_phi *p, uint32_t **d_list, const uint32_t **h_list, uint32_t *len);
+/* Phi::makeModCache  include/phi.cuh:32: out[k-1] = phi(p_1...p_k, k) for
+ * k = 1 .. pi_max-1 (pi_max <= 16), counted on the device where the
+ * primorial is below max and checked against prod (p_i - 1). */
+int cpx_phi_mod_cache(cpx_phi *p, uint32_t pi_max, uint64_t *out);
 int cpx_phi_destroy(cpx_phi *p);
 int cpx_phi_eval(cpx_phi *p, uint32_t x, uint32_t a, uint64_t *out);          /* Phi::phi */
 /* Phi::generateRange(num, a): device u32 array, entry i = #{odd k <= 2i+1 : lpf(k) > p_a},
  * freed with cpx_free. */
 int cpx_phi_range(cpx_phi *p, uint32_t num, uint32_t a, uint32_t **d_out);
 /* gen_d_lpf / gen_d_mu  include/sieve/lpf_mu.cuh:14-21: odd-only arrays for
- * n = bottom + 2i + 1 < top (bottom even); lpf(1) = UINT32_MAX. */
+ * n = bottom + 2i + 1 < top (bottom even); lpf(1) = UINT32_MAX; 32-bit
+ * entries for top < 2^32. */
 int cpx_gen_d_lpf(cpx_ctx *ctx, uint64_t bottom, uint64_t top, uint32_t **d_out);
 int cpx_gen_d_mu(cpx_ctx *ctx, uint64_t bottom, uint64_t top, int8_t **d_out);
+/* gen_d_lpf<uint64_t>  include/sieve/lpf_mu.cuh:14 (src/sieve/lpf_mu.cu:57-87):
+ * 64-bit entries for windows anywhere below 2^64 - 2^34 (a prime n keeps its
+ * whole value); cpx_gen_d_mu takes the same 64-bit windows. */
+int cpx_gen_d_lpf64(cpx_ctx *ctx, uint64_t bottom, uint64_t top, uint64_t **d_out);
 int cpx_free(cpx_ctx *ctx, void *d_ptr);
+/* Device array to host memory (the host halves of gen_h_lpf / gen_h_mu,
+ * include/sieve/lpf_mu.cuh:17,23, copy with it). */
+int cpx_copy_to_host(cpx_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 
 /* ---- batched queries ----------------------------------------------------- */
 typedef struct {

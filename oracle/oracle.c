@@ -181,6 +181,37 @@ void orc_mu_lpf_odd(uint64_t y, int8_t *mu, uint64_t *lpf)
   free(m); free(l);
 }
 
+/* mu / lpf of the odd n = bottom + 2i + 1, i < count (bottom even, the window
+ * below 2^63), each by trial division of its own cofactor by the primes up to
+ * its square root -- no sieve, so it checks the sieved windows of any size
+ * independently (small windows only: ~sqrt(n)/ln sqrt(n) divisions per prime
+ * n).  lpf(1) = UINT64_MAX.  (The 64-bit windows of src/sieve/lpf_mu.cu:57-87,
+ * 153-197.) */
+void orc_mu_lpf_window(uint64_t bottom, uint64_t count, int8_t *mu, uint64_t *lpf)
+{
+  const uint64_t top = bottom + 2 * count;
+  uint64_t r = 1;
+  while ((r + 1) <= top / (r + 1)) r++;          /* isqrt(top) */
+  size_t np;
+  uint32_t *pr = primes_upto(r + 1, &np);
+  for (uint64_t i = 0; i < count; i++) {
+    uint64_t n = bottom + 2 * i + 1, m = n, lp = 0;
+    int par = 0, sq = 0;
+    for (size_t k = 1; k < np && (uint64_t)pr[k] <= m / pr[k]; k++) {   /* odd primes, p^2 <= m */
+      const uint64_t p = pr[k];
+      if (m % p) continue;
+      if (!lp) lp = p;
+      m /= p;
+      par ^= 1;
+      if (m % p == 0) { sq = 1; while (m % p == 0) m /= p; }
+    }
+    if (m > 1) { par ^= 1; if (!lp) lp = m; }
+    lpf[i] = n == 1 ? UINT64_MAX : lp;
+    mu[i] = (int8_t)(sq ? 0 : (par ? -1 : 1));
+  }
+  free(pr);
+}
+
 int64_t orc_mertens(uint64_t n)
 {
   int8_t *m; uint64_t *l;

@@ -43,6 +43,8 @@ def lib():
         L.orc_phi_legendre.argtypes = [U64, ctypes.c_uint32]
         L.orc_mu_lpf_odd.restype = None
         L.orc_mu_lpf_odd.argtypes = [U64, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_mu_lpf_window.restype = None
+        L.orc_mu_lpf_window.argtypes = [U64, U64, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_choose_y.restype = U64
         L.orc_choose_y.argtypes = [U64, U64, ctypes.c_double]
         L.orc_clamp_c.restype = ctypes.c_uint32
@@ -121,6 +123,16 @@ def mu_lpf_odd(y):
     mu = (ctypes.c_int8 * max(n, 1))()
     lpf = (U64 * max(n, 1))()
     lib().orc_mu_lpf_odd(y, mu, lpf)
+    return list(mu[:n]), list(lpf[:n])
+
+
+def mu_lpf_window(bottom, top):
+    """(mu, lpf) lists for the odd n = bottom + 2i + 1 < top (bottom even), by
+    trial division of each n (lpf(1) = 2^64-1)."""
+    n = (top - bottom) // 2
+    mu = (ctypes.c_int8 * max(n, 1))()
+    lpf = (U64 * max(n, 1))()
+    lib().orc_mu_lpf_window(bottom, n, mu, lpf)
     return list(mu[:n]), list(lpf[:n])
 
 

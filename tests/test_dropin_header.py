@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 CALLER = r"""
 #include "cudapix.hpp"
+#include <cmath>
 #include <cstdio>
 // the reference's call sites, unchanged (src/Deleglise-Rivat/deleglise-rivat.cpp:57-67)
 uint64_t dr64(uint64_t x, uint64_t y, uint16_t c) {
@@ -41,6 +42,28 @@ uint64_t sieve_calls(uint64_t y) {
   free(h);
   return n + len + len32 + hl;
 }
+// the reference's V() body over its PrimeArray overloads (src/V.cu:22-56)
+int64_t v_ref(uint64_t x, uint64_t y) {
+  int64_t sum = 0, sum_b = 0;
+  PrimeArray pi, p, q;
+  pi.bottom = 0;
+  pi.top = sqrt(x);
+  pi.d_primes = CudaSieve::getDevicePrimes(pi.bottom, pi.top, pi.len, 0);
+  p.bottom = sqrt(sqrt(x));
+  p.top = sqrt(x / y);
+  q.bottom = 0;
+  q.top = y;
+  sum = V_a(x, pi, p, q);
+  p.bottom = sqrt(x / y);
+  p.top = std::cbrt(x);
+  q.bottom = 0;
+  q.top = y;
+  sum_b = V_b(x, pi, p, q);
+  sum += sum_b;
+  cpx_free(cpx::default_ctx(), pi.d_primes);
+  cpx_free(cpx::default_ctx(), q.d_primes);
+  return sum;
+}
 int main(int argc, char **argv) {
   if (argc > 1) {                       // only with a device
     std::printf("%llu\n", (unsigned long long)sieve_calls(100000));
@@ -49,6 +72,20 @@ int main(int argc, char **argv) {
     std::printf("%llu %llu\n", (unsigned long long)dr64(1000000000000ull, 20000, 6),
                 (unsigned long long)phi.phi(1000, 3));
     (void)t.getCurrent();
+    std::printf("%lld %lld\n", (long long)v_ref(1000000000000ull, 20000), (long long)V(1000000000000ull, 20000));
+    Phi ph(100000, 1000);
+    ph.makeModCache(8);
+    std::printf("%u %u %u %llu\n", ph.get_primeListLength(), ph.get_h_primeList()[0],
+                ph.get_d_lpf() != nullptr, (unsigned long long)ph.mod_cache()[5]);
+    uint64_t *h64 = gen_h_lpf<uint64_t>((uint64_t)1 << 40, ((uint64_t)1 << 40) + 1000);
+    int8_t *hmu = gen_h_mu<uint64_t>((uint64_t)1 << 40, ((uint64_t)1 << 40) + 1000);
+    std::printf("%llu %d\n", (unsigned long long)h64[0], (int)hmu[0]);
+    free(h64);
+    free(hmu);
+    PiTable t2(1000000, 0, 100000);
+    t2.set_pi_base(78498);
+    uint32_t *cur = t2.getCurrent();
+    std::printf("%llu %d\n", (unsigned long long)t2.get_pi_base(), (int)(cur != nullptr && t2.getMaxRange() > 0));
   }
   std::printf("%d\n", cpx_selftest_arith(1, 10000));
   return 0;

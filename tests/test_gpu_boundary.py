@@ -79,4 +79,12 @@ def test_reference_style_caller_device_branch(tmp_path, oracle):
     assert vals[0] == 5 * 9592                          # pi(1e5) five ways
     assert vals[1] + pi_y == 37607912018                # dr64(1e12, 20000, 6) + pi(y)
     assert vals[2] == oracle.phi_legendre(1000, 3)
+    # V through the reference's V() body (PrimeArray V_a + V_b) and V itself
+    assert vals[3] == vals[4] == oracle.V(10 ** 12, 20000, 6)
+    # Phi(1e5, 1000): primes 41..1000, the LPF array, phi of the primorials
+    assert vals[5:9] == [168 - 12, 41, 1, 5760]
+    # gen_h_lpf / gen_h_mu <uint64_t> at n = 2^40 + 1 = 257 * 4278255361
+    assert vals[9:11] == [257, 1]
+    # PiTable::set_pi_base is carried, getMaxRange is positive
+    assert vals[11:13] == [78498, 1]
     assert vals[-1] == 0                                # host self check

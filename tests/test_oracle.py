@@ -80,6 +80,22 @@ def test_mu_lpf_against_sympy(oracle):
         assert mu[i] == (0 if max(f.values()) > 1 else (-1) ** len(f))
 
 
+def test_mu_lpf_window_oracle(oracle):
+    # the trial-division windows against the full sieve below 4e5, and against
+    # sympy's factorisations for 64-bit windows (2^33, 2^44, 2^50)
+    sympy = pytest.importorskip("sympy")
+    mu, lpf = oracle.mu_lpf_odd(400001)
+    for lo, hi in ((0, 20000), (300000, 320002)):
+        wm, wl = oracle.mu_lpf_window(lo, hi)
+        assert wm == mu[lo // 2: hi // 2] and wl == lpf[lo // 2: hi // 2]
+    for lo in (2 ** 33, 2 ** 44 + 1000, 2 ** 50):
+        wm, wl = oracle.mu_lpf_window(lo, lo + 400)
+        for i in range(0, 200, 7):
+            f = sympy.factorint(lo + 2 * i + 1)
+            assert wl[i] == min(f)
+            assert wm[i] == (0 if max(f.values()) > 1 else (-1) ** len(f))
+
+
 def test_phi_legendre_equals_definition(oracle):
     rnd = random.Random(5)
     for _ in range(40):
