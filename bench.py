@@ -144,34 +144,112 @@ def profiled_traffic(workload):
     return None
 
 
-def cpu_baseline(xs, budget_s=20.0):
-    """Oracle (C + OpenMP) on a stratified sample of the workload."""
+def host_info():
+    """lscpu model, logical CPUs, the CPUs this process may run on (nproc) and
+    the OpenMP thread count the CPU baseline uses."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)",
+                     "CPU max MHz"):
+                info["lscpu " + k] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    info["threads"] = int(env) if env.isdigit() and int(env) > 0 else info["nproc"]
+    return info
+
+
+def median_time(fn, reps=3, warm=True):
+    """1 warm-up call (optional), then the median wall time of `reps` calls
+    (steady clock); returns (median seconds, value of the last call)."""
+    if warm:
+        fn()
+    ts, v = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        v = fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], v
+
+
+def cpu_alpha(x):
+    """y = alpha * cbrt(x) for the CPU oracle: the hard-leaf walk dominates on
+    the CPU, so alpha grows with x (chosen by timing the sample strata)."""
+    return max(1.0, min(12.0, math.log10(x) - 5.0))
+
+
+def cpu_baseline(xs, full=False, budget_s=90.0):
+    """The oracle (C + OpenMP, rebuilt here with -O3 -march=native) on the
+    host: 1 warm-up and the median of 3 for every timing.
+      * queries/s on a stratified sample of the 512-query list (the median x
+        of each of 8 strata of 64 sorted queries; a stratum's time stands for
+        its 64 queries);
+      * C1 (x = 1e12, alpha 2, c = 6): P2, ordinary, trivial, easy and hard
+        terms, and the brute segmented-sieve count of primes below 1e12 timed
+        on the window [1e12 - 2^33, 1e12] and scaled by 1e12 / 2^33 (with
+        full: the whole count);
+      * C2 (x = 1e16, alpha 2): ordinary and trivial (with full: P2 too)."""
     from oracle import oracle as O
+    flags = O.use_native()
+    hi = host_info()
+    th = hi["threads"]
     srt = sorted(xs)
-    k = 16
-    idx = [int((i + 0.5) * len(srt) / k) for i in range(k)]
-    sample = [srt[i] for i in idx]
-    threads = os.cpu_count() or 1
-    t_sum = 0.0
-    done = 0
-    est_total = 0.0
-    for i, x in enumerate(sample):
-        y = O.choose_y(x, max(1.0, min(8.0, 0.25 * math.log10(x) - 1.0)))
-        t0 = time.time()
-        O.pi_dr(x, y, 6, threads)
-        dt = time.time() - t0
-        t_sum += dt
-        done += 1
-        est_total += dt * (len(srt) / k)
-        if t_sum > budget_s:
-            # remaining strata are larger x: scale by the last measured time
-            est_total += dt * (len(srt) / k) * (k - done) * 2.0
-            break
-    return {"value": len(srt) / est_total, "unit": "queries/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d of 16 strata (1 x per 32-query stratum of the sorted 512-query list), "
-                      "%.1f s of CPU; oracle P2/ordinary/easy OpenMP, hard-leaf walk 1 thread%s"
-                      % (done, t_sum, "" if done == k else ", untimed strata extrapolated x2")}
+    k = 8
+    per = len(srt) // k
+    sample = [srt[i * per + per // 2] for i in range(k)]
+    O.pi_dr(sample[0], O.choose_y(sample[0], cpu_alpha(sample[0])), 6, th)      # warm-up
+    est, spent, strata = 0.0, 0.0, []
+    for x in sample:
+        y = O.choose_y(x, cpu_alpha(x))
+        reps = 3 if spent < budget_s else 1
+        t, r = median_time(lambda: O.pi_dr(x, y, 6, th), reps=reps, warm=False)
+        spent += t * reps
+        est += t * per
+        strata.append({"x": x, "y": y, "s": round(t, 4), "reps": reps})
+    out = {"value": round(len(srt) / est, 4), "unit": "queries/s", "cores": th, "kind": "port",
+           "sample": "median x of each of 8 strata of the sorted 512-query list, each timed as the "
+                     "median of 3 (1 warm-up before the first), alpha = clamp(log10 x - 5, 1, 12); "
+                     "%.1f s of CPU" % spent,
+           "build": "gcc " + flags, "host": hi, "strata": strata}
+
+    x1 = 10 ** 12
+    y1 = O.choose_y(x1, 2.0)
+    c1 = {"x": x1, "y": y1, "c": 6}
+    for name, fn in (("p2", lambda: O.P2(x1, y1, th)), ("ordinary", lambda: O.Ordinary(x1, y1, 6, th)),
+                     ("trivial", lambda: O.S2_trivial(x1, y1, 6)), ("easy", lambda: O.V(x1, y1, 6, th)),
+                     ("hard", lambda: O.S3(x1, y1, 6, th)), ("pi_dr", lambda: O.pi_dr(x1, y1, 6, th))):
+        t, v = median_time(fn)
+        c1[name + "_ms"] = round(t * 1000, 3)
+    c1["p2_ord_triv_ms"] = round(c1["p2_ms"] + c1["ordinary_ms"] + c1["trivial_ms"], 3)
+    if v["pi"] != 37607912018:
+        raise SystemExit("oracle pi(1e12) wrong: %d" % v["pi"])
+    if full:
+        t, v = median_time(lambda: O.count_primes(0, x1 + 1, th), reps=3)
+        if v != 37607912018:
+            raise SystemExit("brute count below 1e12 wrong: %d" % v)
+        c1["brute_ms"] = round(t * 1000, 1)
+        c1["brute_sample"] = "the whole count"
+    else:
+        w = 1 << 33
+        t, _ = median_time(lambda: O.count_primes(x1 - w, x1 + 1, th))
+        c1["brute_ms"] = round(t * 1000 * x1 / w, 1)
+        c1["brute_sample"] = "window [1e12 - 2^33, 1e12] in %.3f s (median of 3), scaled by 1e12/2^33" % t
+    out["c1"] = c1
+
+    x2 = 10 ** 16
+    y2 = O.choose_y(x2, 2.0)
+    c2 = {"x": x2, "y": y2, "c": 6}
+    for name, fn in (("ordinary", lambda: O.Ordinary(x2, y2, 6, th)), ("trivial", lambda: O.S2_trivial(x2, y2, 6))) \
+            + ((("p2", lambda: O.P2(x2, y2, th)),) if full else ()):
+        t, _ = median_time(fn)
+        c2[name + "_ms"] = round(t * 1000, 3)
+    out["c2"] = c2
+    return out
 
 
 def configs_run(args):
@@ -284,6 +362,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline: the whole brute count below 1e12 and P2 at 1e16 too")
+    ap.add_argument("--cpu-only", action="store_true", help="print only the CPU baseline object and exit")
     ap.add_argument("--serve-n", type=int, default=400)
     ap.add_argument("--serve-lambdas", default="200,2000")
     ap.add_argument("--serve-xmax", type=float, default=1e16)
@@ -308,6 +389,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_only:
+        print(json.dumps(cpu_baseline(load_json("queries_c3.json")["x"], full=args.cpu_full)), flush=True)
+        return
     if args.profile:
         profile_run(args)
         return
@@ -432,6 +516,14 @@ def main():
                               "lds_algorithmic_GB": round(w16 / 1e9, 2),
                               "pi": t["pi"]}
         log("pi(1e16) = %d in %.1f ms" % (t["pi"], d16 * 1000))
+        # C1: pi(1e12) at alpha 2 (y = 20000), c = 6, warm context, median of 3
+        c1 = C.Context(device=0)
+        ms1, v1 = median_time(lambda: c1.pi(10 ** 12, y=20000, c=6))
+        c1.close()
+        if v1 != 37607912018:
+            raise SystemExit("pi(1e12) wrong: %d" % v1)
+        out["c1_gpu_ms"] = round(ms1 * 1000, 3)
+        log("pi(1e12) at y=20000: %.2f ms" % (ms1 * 1000))
 
     # ---- served Poisson streams
     if not args.no_serve:
@@ -516,7 +608,7 @@ def main():
                                          / (lds_peak / 1e9), 4)
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(xs)
+        out["cpu_baseline"] = cpu_baseline(xs, full=args.cpu_full)
         log("cpu baseline: %s" % out["cpu_baseline"])
 
     ctx.close()

@@ -499,9 +499,13 @@ void orc_easy(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, u
  * Restates the staged sieve of src/sieve/S2_hard_device.cu:264-355: the odd
  * numbers of [1, z] start crossed by p_1..p_c; at stage b the leaves of p_b
  * read phi(n, b-1) = survivors <= n, then p_b is crossed off.  Leaves of one
- * stage are walked with m descending, so n = x/(m p) ascends and one running
- * count per stage suffices (the reference's countUpSieve walk). */
-void orc_hard(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, uint64_t out[2])
+ * stage are walked with m descending, so n = x/(m p) ascends and
+ * one running count per stage suffices (the reference's countUpSieve walk).
+ * OpenMP: each thread owns a contiguous run of bitmap words; per stage the
+ * threads count their survivors, take the counts of the runs below theirs as
+ * the carry into their own run, evaluate the leaves whose n falls in their run
+ * and cross p off in it (the per-block carry of src/sieve/S2_hard_host.cu:115-191). */
+void orc_hard(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, uint64_t out[2])
 {
   u128 x = mk128(xhi, xlo);
   params_t P; make_params(&P, x, y, c);
@@ -516,39 +520,70 @@ void orc_hard(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, uint64_t out[2
   uint64_t nbits = z / 2 + 1;                        /* bit i <-> 2i+1, covers 1..z(+1) */
   uint64_t nw = (nbits + 63) / 64;
   uint64_t *bm = (uint64_t *)malloc(nw * sizeof(uint64_t));
-  for (uint64_t w = 0; w < nw; w++) {
-    uint64_t v = 0;
-    for (int t = 0; t < 64; t++) {
-      uint64_t n = 2 * (w * 64 + t) + 1;
-      int ok = 1;
-      for (uint32_t i = 1; i < c; i++) if (n % SMALLP[i] == 0) { ok = 0; break; }
-      if (ok) v |= 1ull << t;
+  int T = 1;
+#ifdef _OPENMP
+  T = threads > 0 ? threads : omp_get_max_threads();
+#endif
+  uint64_t *cnt = (uint64_t *)calloc((size_t)T + 1, sizeof(uint64_t));
+  i128 *part = (i128 *)calloc((size_t)T + 1, sizeof(i128));
+#pragma omp parallel num_threads(T)
+  {
+    int t = 0, nt = 1;
+#ifdef _OPENMP
+    t = omp_get_thread_num(); nt = omp_get_num_threads();
+#endif
+    uint64_t per = (nw + (uint64_t)nt - 1) / (uint64_t)nt;
+    uint64_t w0 = (uint64_t)t * per; if (w0 > nw) w0 = nw;
+    uint64_t w1 = w0 + per; if (w1 > nw) w1 = nw;
+    uint64_t bs = 64 * w0, be = 64 * w1;             /* own bits [bs, be) */
+    for (uint64_t w = w0; w < w1; w++) {
+      uint64_t v = 0;
+      for (int k = 0; k < 64; k++) {
+        uint64_t n = 2 * (w * 64 + k) + 1;
+        int ok = 1;
+        for (uint32_t i = 1; i < c; i++) if (n % SMALLP[i] == 0) { ok = 0; break; }
+        if (ok) v |= 1ull << k;
+      }
+      bm[w] = v;
     }
-    bm[w] = v;
+    i128 s3 = 0;
+    for (uint64_t b = c + 1; b <= B; b++) {
+      uint64_t p = pr[b - 1];
+      u128 xp = x / p;
+      uint64_t own = 0;
+      for (uint64_t w = w0; w < w1; w++) own += (uint64_t)__builtin_popcountll(bm[w]);
+      cnt[t] = own;
+#pragma omp barrier
+      uint64_t run = 0;                              /* survivors in words [0, w) */
+      for (int i = 0; i < t; i++) run += cnt[i];
+      /* leaves with bit((x/p)/m) in [bs, be): x/(p m) >= 2 bs + 1 and < 2 be + 1 */
+      uint64_t mlo = y / p;                          /* m > y/p */
+      u128 top = xp / (2 * (u128)bs + 1), bot = xp / (2 * (u128)be + 1);
+      uint64_t mhi = top < (u128)y ? (uint64_t)top : y;
+      if (bot > (u128)mlo) mlo = (uint64_t)bot;
+      uint64_t w = w0;
+      for (uint64_t m = mhi; m > mlo; m--) {
+        if (!(m & 1) || !mu[m] || lpf[m] <= p) continue;
+        uint64_t n = (uint64_t)(xp / m);
+        uint64_t bit = (n - 1) / 2;                  /* largest odd <= n */
+        uint64_t tw = bit / 64;
+        while (w < tw) run += (uint64_t)__builtin_popcountll(bm[w++]);
+        uint64_t msk = (bit % 64 == 63) ? ~0ull : ((2ull << (bit % 64)) - 1);
+        uint64_t phi = run + (uint64_t)__builtin_popcountll(bm[tw] & msk);
+        s3 -= mu[m] * (i128)phi;
+      }
+      /* cross off odd multiples of p (p itself included) among own bits below nbits */
+      uint64_t i0 = (p - 1) / 2, lim = be < nbits ? be : nbits;
+      if (i0 < bs) i0 += (bs - i0 + p - 1) / p * p;
+      for (uint64_t i = i0; i < lim; i += p) bm[i / 64] &= ~(1ull << (i % 64));
+#pragma omp barrier
+    }
+    part[t] = s3;
   }
   i128 s3 = 0;
-  for (uint64_t b = c + 1; b <= B; b++) {
-    uint64_t p = pr[b - 1];
-    u128 xp = x / p;
-    uint64_t mlo = y / p;                            /* m > y/p */
-    uint64_t w = 0, run = 0;                         /* survivors in words [0, w) */
-    for (uint64_t m = y; m > mlo; m--) {
-      if (!(m & 1) || !mu[m] || lpf[m] <= p) continue;
-      uint64_t n = (uint64_t)(xp / m);
-      uint64_t bit = (n - 1) / 2;                    /* largest odd <= n */
-      uint64_t tw = bit / 64;
-      while (w < tw) run += (uint64_t)__builtin_popcountll(bm[w++]);
-      uint64_t msk = (bit % 64 == 63) ? ~0ull : ((2ull << (bit % 64)) - 1);
-      uint64_t phi = run + (uint64_t)__builtin_popcountll(bm[tw] & msk);
-      s3 -= mu[m] * (i128)phi;
-    }
-    /* cross off odd multiples of p (p itself included) */
-    for (uint64_t n = p; n <= 2 * (nbits - 1) + 1; n += 2 * p) {
-      uint64_t i = (n - 1) / 2; bm[i / 64] &= ~(1ull << (i % 64));
-    }
-  }
+  for (int i = 0; i <= T; i++) s3 += part[i];
   put128((u128)s3, out);
-  free(bm); free(mu); free(lpf); free(pr);
+  free(part); free(cnt); free(bm); free(mu); free(lpf); free(pr);
 }
 
 /* ------------------------------------------------------------ assembly */
@@ -564,7 +599,7 @@ void orc_pi_dr(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, 
   orc_ordinary(xhi, xlo, y, c, threads, t); s0 = (i128)mk128(t[1], t[0]);
   orc_trivial(xhi, xlo, y, c, t);           tr = (i128)mk128(t[1], t[0]);
   orc_easy(xhi, xlo, y, c, threads, t);     v = (i128)mk128(t[1], t[0]);
-  orc_hard(xhi, xlo, y, c, t);              s3 = (i128)mk128(t[1], t[0]);
+  orc_hard(xhi, xlo, y, c, threads, t);     s3 = (i128)mk128(t[1], t[0]);
   orc_p2(xhi, xlo, y, threads, t);          p2 = (i128)mk128(t[1], t[0]);
   size_t np; uint32_t *pr = primes_upto(y, &np); free(pr);
   i128 pix = s0 + tr + v + s3 + (i128)np - 1 - p2;

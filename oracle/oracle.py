@@ -22,46 +22,60 @@ def build():
     subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
 
 
+def use_native():
+    """Rebuild the oracle with -O3 -march=native on this machine and route
+    every later call through that build (for timing the CPU baseline; the
+    portable build stays the one the tests load).  Returns the compile flags."""
+    global _lib
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle_native.so"], check=True)
+    _lib = _load(os.path.join(_HERE, "liboracle_native.so"))
+    return "-O3 -march=native -fopenmp"
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_SO):
             build()
-        L = ctypes.CDLL(_SO)
-        for name in ("orc_isqrt", "orc_icbrt", "orc_iroot4"):
-            getattr(L, name).restype = U64
-            getattr(L, name).argtypes = [U64, U64]
-        L.orc_count_primes.restype = U64
-        L.orc_count_primes.argtypes = [U64, U64, ctypes.c_int]
-        L.orc_primes.restype = ctypes.c_size_t
-        L.orc_primes.argtypes = [U64, ctypes.c_void_p, ctypes.c_size_t]
-        L.orc_mertens.restype = ctypes.c_int64
-        L.orc_mertens.argtypes = [U64]
-        L.orc_phi_brute.restype = U64
-        L.orc_phi_brute.argtypes = [U64, ctypes.c_uint32]
-        L.orc_phi_legendre.restype = U64
-        L.orc_phi_legendre.argtypes = [U64, ctypes.c_uint32]
-        L.orc_mu_lpf_odd.restype = None
-        L.orc_mu_lpf_odd.argtypes = [U64, ctypes.c_void_p, ctypes.c_void_p]
-        L.orc_mu_lpf_window.restype = None
-        L.orc_mu_lpf_window.argtypes = [U64, U64, ctypes.c_void_p, ctypes.c_void_p]
-        L.orc_choose_y.restype = U64
-        L.orc_choose_y.argtypes = [U64, U64, ctypes.c_double]
-        L.orc_clamp_c.restype = ctypes.c_uint32
-        L.orc_clamp_c.argtypes = [U64, U64, ctypes.c_uint32]
-        out2 = ctypes.POINTER(U64)
-        L.orc_p2.argtypes = [U64, U64, U64, ctypes.c_int, out2]
-        L.orc_ordinary.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
-        L.orc_trivial.argtypes = [U64, U64, U64, ctypes.c_uint32, out2]
-        L.orc_easy.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
-        L.orc_easy_part.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, out2]
-        L.orc_hard.argtypes = [U64, U64, U64, ctypes.c_uint32, out2]
-        L.orc_pi_dr.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
-        for n in ("orc_p2", "orc_ordinary", "orc_trivial", "orc_easy", "orc_easy_part", "orc_hard",
-              "orc_pi_dr"):
-            getattr(L, n).restype = None
-        _lib = L
+        _lib = _load(_SO)
     return _lib
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    for name in ("orc_isqrt", "orc_icbrt", "orc_iroot4"):
+        getattr(L, name).restype = U64
+        getattr(L, name).argtypes = [U64, U64]
+    L.orc_count_primes.restype = U64
+    L.orc_count_primes.argtypes = [U64, U64, ctypes.c_int]
+    L.orc_primes.restype = ctypes.c_size_t
+    L.orc_primes.argtypes = [U64, ctypes.c_void_p, ctypes.c_size_t]
+    L.orc_mertens.restype = ctypes.c_int64
+    L.orc_mertens.argtypes = [U64]
+    L.orc_phi_brute.restype = U64
+    L.orc_phi_brute.argtypes = [U64, ctypes.c_uint32]
+    L.orc_phi_legendre.restype = U64
+    L.orc_phi_legendre.argtypes = [U64, ctypes.c_uint32]
+    L.orc_mu_lpf_odd.restype = None
+    L.orc_mu_lpf_odd.argtypes = [U64, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_mu_lpf_window.restype = None
+    L.orc_mu_lpf_window.argtypes = [U64, U64, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_choose_y.restype = U64
+    L.orc_choose_y.argtypes = [U64, U64, ctypes.c_double]
+    L.orc_clamp_c.restype = ctypes.c_uint32
+    L.orc_clamp_c.argtypes = [U64, U64, ctypes.c_uint32]
+    out2 = ctypes.POINTER(U64)
+    L.orc_p2.argtypes = [U64, U64, U64, ctypes.c_int, out2]
+    L.orc_ordinary.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+    L.orc_trivial.argtypes = [U64, U64, U64, ctypes.c_uint32, out2]
+    L.orc_easy.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+    L.orc_easy_part.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, out2]
+    L.orc_hard.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+    L.orc_pi_dr.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+    for n in ("orc_p2", "orc_ordinary", "orc_trivial", "orc_easy", "orc_easy_part", "orc_hard",
+              "orc_pi_dr"):
+        getattr(L, n).restype = None
+    return L
 
 
 def _split(x):
@@ -168,8 +182,8 @@ def V_b(x, y, c=6, threads=0):
     return _term(lib().orc_easy_part, *_split(x), y, c, threads, 2)
 
 
-def S3(x, y, c=6):
-    return _term(lib().orc_hard, *_split(x), y, c)
+def S3(x, y, c=6, threads=0):
+    return _term(lib().orc_hard, *_split(x), y, c, threads)
 
 
 def pi_dr(x, y, c=6, threads=0):
