@@ -113,7 +113,11 @@ def main(argv=None):
         ap.error("x, y must be non-negative and c in 1..8")
 
     import cudapix_amd as C
-    ctx = C.Context(device=a.device, c=c, alpha=a.alpha)
+    try:
+        ctx = C.Context(device=a.device, c=c, alpha=a.alpha)
+    except C.CpxError as e:
+        print(json.dumps({"error": str(e), "code": e.code}), flush=True)
+        return 1
     try:
         if a.batch is not None:
             res = ctx.pi_batch(xs, terms=True)
