@@ -311,7 +311,8 @@ def profile_run(args):
     """One workload only, for profiler runs: `single` = pi(x) on a warm
     context (default shape), `batch` = the 512-query batched step (with
     --tuning as the timed run's autotune chose it), `copy` = a
-    device-to-device copy of --copy-mib MiB (the byte-counter calibration)."""
+    device-to-device copy of --copy-mib MiB (the byte-counter calibration),
+    `probe` = the LDS and VALU peak microbenchmarks."""
     if args.profile == "copy":
         import torch
         n = args.copy_mib << 20
@@ -326,6 +327,13 @@ def profile_run(args):
         return
     import cudapix_amd as C
     ctx = C.Context(device=0)
+    if args.profile == "probe":
+        for _ in range(args.reps):
+            lds, valu = ctx.probe_peaks()
+            print(json.dumps({"profile": "probe", "lds_GBs": round(lds / 1e9, 1),
+                              "valu_lane_ops": round(valu / 1e12, 3)}), flush=True)
+        ctx.close()
+        return
     if args.profile == "single":
         x = int(args.x)
         ctx.pi(x)
@@ -370,7 +378,7 @@ def main():
     ap.add_argument("--serve-xmax", type=float, default=1e16)
     ap.add_argument("--serve-full", action="store_true",
                     help="every stream of the served fixture: lambda 50..2000, 2000 requests, x < 1e17")
-    ap.add_argument("--profile", choices=["single", "batch", "copy"],
+    ap.add_argument("--profile", choices=["single", "batch", "copy", "probe"],
                     help="run only one workload --reps times (for rocprofv3) and exit")
     ap.add_argument("--x", type=float, default=1e16, help="x of --profile single")
     ap.add_argument("--reps", type=int, default=3)

@@ -225,10 +225,15 @@ __device__ __forceinline__ void seg_fill_wheel(uint32_t *lds, uint32_t K, uint32
                                                const PiEntry *__restrict__ pat, uint32_t P)
 {
   const uint32_t t = threadIdx.x;
-  uint32_t base = (uint32_t)((lo >> 6) % P);
-  uint32_t W = K * T;
+  const uint32_t W = K * T;
+  // the pattern index steps by T mod P instead of a division per word, so
+  // the loads of the unrolled loop go out together
+  const uint32_t step = T % P;
+  uint32_t idx = (uint32_t)(((lo >> 6) + t) % P);
+#pragma unroll 8
   for (uint32_t g = t; g < W; g += T) {
-    uint32_t idx = (base + g) % P;
     lds[seg_addr(g, K, T)] = pat[idx].bits;
+    idx += step;
+    if (idx >= P) idx -= P;
   }
 }

@@ -281,6 +281,28 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
   }
 }
 
+// Clear bits j, j + p, ... below lim of a thread's own words, adding to d
+// the bits that were still set; returns the first multiple at or above lim.
+// Four and-with-returns are issued before their results are used, so each
+// lane keeps four LDS round trips in flight (the LDS applies one lane's
+// operations in order, so multiples that share a word still see each
+// other).  ubfe and the shift take the bit offset modulo 32.
+__device__ __forceinline__ uint32_t cross_own_run(uint32_t *own, uint32_t j, uint32_t lim, uint32_t p,
+                                                  uint32_t &d)
+{
+  for (; j + 3 * p < lim; j += 4 * p) {
+    const uint32_t j1 = j + p, j2 = j1 + p, j3 = j2 + p;
+    const uint32_t o0 = atomicAnd(&own[j >> 5], ~(1u << (j & 31u)));
+    const uint32_t o1 = atomicAnd(&own[j1 >> 5], ~(1u << (j1 & 31u)));
+    const uint32_t o2 = atomicAnd(&own[j2 >> 5], ~(1u << (j2 & 31u)));
+    const uint32_t o3 = atomicAnd(&own[j3 >> 5], ~(1u << (j3 & 31u)));
+    d += __builtin_amdgcn_ubfe(o0, j, 1u) + __builtin_amdgcn_ubfe(o1, j1, 1u) +
+         __builtin_amdgcn_ubfe(o2, j2, 1u) + __builtin_amdgcn_ubfe(o3, j3, 1u);
+  }
+  for (; j < lim; j += p) d += __builtin_amdgcn_ubfe(atomicAnd(&own[j >> 5], ~(1u << (j & 31u))), j, 1u);
+  return j;
+}
+
 // First leafy item of the chunk at or after item k (HARD_SCH if none).
 template <int T, int K>
 __device__ __forceinline__ uint32_t hard_next_leafy(const SegShared<T, K> &sh, uint32_t k)
@@ -699,7 +721,7 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
         // ---- leaves of p whose n falls in this wave's bits (state: crossed
         //      by p_1 .. p_{b-1}), all members' in one list; the wave
         //      publishes its own prefix, no other wave is involved
-        {
+        if (ntot > rep * CPX_WAVE) {                    // (a wave with no entries reads none)
           const uint32_t incl = wave_incl_scan(cnt);
           sh.ipre[t] = (incl - cnt) | (cnt_lo << CountPack<K>::PRE_BITS);
         }
@@ -758,36 +780,36 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
         const uint32_t oseg = sh.u.st.oseg[k];
         uint32_t tb = b0;
         if (p <= b0) {                                    // p < 2^20 here
-          const uint32_t qe = (uint32_t)(fb0 * sh.u.st.rinv[k]);
-          int32_t r = (int32_t)(b0 - __umul24(qe, p));
-          if (r < 0) r += (int32_t)p;
-          else if (r >= (int32_t)p) r -= (int32_t)p;
+          // b0 mod p in f32: b0 < 2^20 and q p < 2^21 are exact, so the fma
+          // gives b0 - q p exactly, with q the quotient or one off
+          const float q = truncf(fb0 * sh.u.st.rinv[k]);
+          int32_t r = (int32_t)fmaf(-q, (float)p, fb0);
+          r += r < 0 ? (int32_t)p : 0;
+          r -= r >= (int32_t)p ? (int32_t)p : 0;
           tb = (uint32_t)r;
         }
         uint32_t j = oseg >= tb ? oseg - tb : oseg + p - tb;
-        const uint32_t a0 = t * (K + 1);
+        uint32_t *const own = lds + t * (K + 1);
         uint32_t wd;
         // (one LDS and-with-return per multiple: the old word says whether
         // the bit was still set; the words are the thread's own)
         if (p > BPT) {                                    // at most one multiple per thread
           uint32_t hit = 0;
           if (j < BPT) {
-            const uint32_t old = atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31u)));
+            const uint32_t old = atomicAnd(&own[j >> 5], ~(1u << (j & 31u)));
             hit = (old >> (j & 31u)) & 1u;
           }
           cnt -= hit;
           cnt_lo -= hit & (j < BPT / 2 ? 1u : 0u);
           wd = (uint32_t)__popcll(__ballot(hit));
         } else {
-          uint32_t d = 0, dlo = 0;
-          for (; j < BPT; j += p) {
-            const uint32_t old = atomicAnd(&lds[a0 + (j >> 5)], ~(1u << (j & 31u)));
-            const uint32_t h = (old >> (j & 31u)) & 1u;
-            d += h;
-            dlo += j < BPT / 2 ? h : 0u;
-          }
+          // the first half's multiples, then the second's: the removals of
+          // the first half are the running sum at the switch
+          uint32_t d = 0;
+          j = cross_own_run(own, j, BPT / 2, p, d);
+          cnt_lo -= d;
+          cross_own_run(own, j, BPT, p, d);
           cnt -= d;
-          cnt_lo -= dlo;
           wd = wave_sum(d);
         }
         if (lane == 0) sh.u.st.dw[k][wv] = (uint16_t)wd;   // this wave's removals
