@@ -14,8 +14,11 @@ Beside it, in the same run:
   * plain: the same 512 x solved one at a time, each with its own table build
     (the reference's per-term, per-query table rebuilds), and plain_warm with
     tables kept resident;  speedup = value / plain qps;
-  * served: a Poisson stream (tests/golden/queries_c4.json) through the
-    continuous-batching server, p50/p99 latency and queries/s;
+  * served: Poisson streams (tests/golden/queries_c4.json) through the server
+    in both modes -- continuous batching into one streaming launch, and one
+    batch launch per admitted FIFO prefix -- p50/p99 latency and queries/s,
+    every served value checked against the same x solved alone, beside the
+    plain one-at-a-time rate on the same x list;
   * single: pi(1e16) wall time;
   * roofline of the persistent segment kernel (HIP events on its stream);
   * cpu_baseline: the oracle's C/OpenMP restatement of the reference terms on
@@ -67,6 +70,11 @@ def primes_upto(n):
 
 GROUP = 8   # queries per shared segment sieve (the library's group capacity)
 
+# Tuning of the batched runs (segment threads, words per thread, alpha factor,
+# wheel primes): the fastest of the autotuner's choices on the 512-query
+# workload, and the one the committed counter profiles were taken at.
+BENCH_TUNING = "512,32,0.3,8"
+
 
 def sieve_groups(queries, primes):
     """The library's query groups: increasing x, the same wheel c, at most
@@ -114,34 +122,70 @@ def algorithmic_lds_bytes(queries, primes):
     return tot
 
 
-def profiled_valu(workload):
-    """Measured VALU issue fraction of k_segments (SQ_INSTS_VALU over the
-    measured int32 issue peak) from the committed counter summary, or None."""
+def profiled_counters(workload):
+    """Per-launch k_segments figures from the committed rocprofv3 --pmc
+    summary of the same workload and tuning: lines `key: value` (tuning,
+    valu_instr_per_launch, salu_instr_per_launch, lds_bytes_per_launch,
+    hbm_bytes_per_launch).  Empty if the summary is absent."""
     path = os.path.join(ROOT, "profiles", "k_segments_%s_pmc.md" % workload)
+    out = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("| VALU issue rate |"):
-                    pct = line.split("=")[1].split("%")[0]
-                    return round(float(pct) / 100.0, 4)
-    except (OSError, IndexError, ValueError):
+                k, sep, v = line.partition(":")
+                if sep and k.strip() in ("tuning", "valu_instr_per_launch", "salu_instr_per_launch",
+                                         "lds_bytes_per_launch", "hbm_bytes_per_launch"):
+                    v = v.strip()
+                    out[k.strip()] = v if k.strip() == "tuning" else float(v)
+    except (OSError, ValueError):
         pass
-    return None
+    return out
 
 
-def profiled_traffic(workload):
-    """HBM bytes per k_segments launch from the committed rocprofv3 --pmc
-    summary of the same workload (FETCH_SIZE and WRITE_SIZE, scaled by the
-    copy-kernel calibration recorded there), or None."""
-    path = os.path.join(ROOT, "profiles", "k_segments_%s_pmc.md" % workload)
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("hbm_bytes_per_launch:"):
-                    return int(float(line.split(":", 1)[1]))
-    except OSError:
-        pass
-    return None
+# Documented MI355X rates the roofline divides by (256 CUs at 2.4 GHz peak
+# engine clock):
+#  * VALU issue: the published FP32 vector peak, 157.3 TFLOP/s = 256 CUs x
+#    2.4 GHz x 128 FMA lanes per clock, i.e. two wave64 32-bit VALU
+#    instructions per CU per clock: 1228.8 G wave64 instructions/s;
+#  * LDS: 256 bytes per CU per clock on CDNA 4 (twice CDNA 3's 128, per AMD's
+#    CDNA 4 white paper), 157.3 TB/s;
+#  * HBM3E: 8 TB/s.
+CU_COUNT = 256
+CLOCK_GHZ = 2.4
+VALU_PEAK_G = 2 * CU_COUNT * CLOCK_GHZ          # G wave64 instructions/s
+LDS_PEAK_GBS = 256 * CU_COUNT * CLOCK_GHZ       # GB/s
+
+
+def hbm_model_bytes(terms, primes, seg_bits):
+    """W_HBM of one batch launch (a model, labelled as such): 4 B per hard-leaf
+    candidate (the packed mu*lpf word of each odd m in (y/p, y], or the prime
+    list entry when p^2 > y), 8 B per easy-leaf pair (one pi-table entry),
+    8 B per P2 prime (one pi-table entry at x/p), and 8 B per wheel word
+    filled into each segment of each query group (seg_bits odd numbers per
+    segment; replicas of a segment not counted).  Tables are resident and
+    not rebuilt inside the launch."""
+    import numpy as np
+    nseg = sum(-(-((max(t["x"] // t["y"] for t in g) + 1) // 2) // seg_bits)
+               for g in sieve_groups(terms, primes))
+    pi_at = lambda v: int(np.searchsorted(primes, v, side="right"))
+    l_hard = l_easy = n_p2 = 0
+    for t in terms:
+        x, y, c = t["x"], t["y"], int(t["c"])
+        q4, c3, sq = iroot(x, 4), iroot(x, 3), iroot(x, 2)
+        ps = primes[c:pi_at(q4)].astype(np.int64)
+        dense = ps[ps * ps <= y]
+        l_hard += int(np.sum((y - y // dense + 1) // 2))
+        for p in ps[ps * ps > y]:
+            l_hard += max(0, pi_at(y) - pi_at(max(int(p), y // int(p))))
+        for p in primes[pi_at(q4):pi_at(c3)]:
+            hi = min(y, x // (int(p) * int(p)))
+            if hi > p:
+                l_easy += pi_at(hi) - pi_at(int(p))
+        n_p2 += max(0, pi_at(sq) - pi_at(y))
+    wheel = nseg * seg_bits // 32
+    w = 4 * l_hard + 8 * l_easy + 8 * n_p2 + 8 * wheel
+    return w, {"hard_candidates": l_hard, "easy_pairs": l_easy, "p2_primes": n_p2,
+               "group_segments": nseg, "wheel_words": wheel}
 
 
 def host_info():
@@ -346,9 +390,11 @@ def profile_run(args):
               flush=True)
     else:
         xs = load_json("queries_c3.json")["x"]
-        if args.tuning:
+        if args.tuning != "auto":
             t, k, a, c = args.tuning.split(",")
             ctx.set_tuning(int(t), int(k), float(a), int(c))
+        else:
+            ctx.autotune(xs)
         ref = ctx.pi_batch(xs)
         ms = []
         for _ in range(args.reps):
@@ -386,8 +432,9 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="time the single-query configurations (1e12, the 1e16 alpha x segment sweep, "
                          "1e20 at two alphas) as markdown rows and exit")
-    ap.add_argument("--tuning", default="",
-                    help="threads,kwords,alpha_scale,c for --profile batch (as the timed run's autotune chose)")
+    ap.add_argument("--tuning", default=BENCH_TUNING,
+                    help="threads,kwords,alpha_scale,c of the batched runs and --profile batch, or 'auto' "
+                         "(autotune; the committed counters then may not match)")
     args = ap.parse_args()
     if args.serve_full:
         args.serve_lambdas, args.serve_n, args.serve_xmax = "50,200,500,1000,2000", 2000, 1e17
@@ -412,12 +459,20 @@ def main():
     n = len(xs)
     ctx = C.Context(device=0)
 
-    # ---- warmup (builds resident tables, autotunes the segment shape)
-    log("warmup: autotune + %d batched steps" % args.warmup)
+    # ---- warmup (builds resident tables; the segment shape, alpha factor and
+    # wheel are pinned to the tuning the committed counter profiles were
+    # taken at, or autotuned with --tuning auto)
+    log("warmup: tuning %s + %d batched steps" % (args.tuning, args.warmup))
     t0 = time.time()
-    shape = ctx.autotune(xs)
+    if args.tuning == "auto":
+        ctx.autotune(xs)
+    else:
+        tt, tk, ta, tc = args.tuning.split(",")
+        ctx.set_tuning(int(tt), int(tk), float(ta), int(tc))
     tuning = ctx.tuning()
-    log("autotuned %s in %.1fs" % (tuning, time.time() - t0))
+    shape = (tuning["seg_threads"], tuning["seg_kwords"])
+    tuning_key = "%d,%d,%g,%d" % (shape[0], shape[1], tuning["alpha_scale"], tuning["c"])
+    log("tuning %s (%.1fs)" % (tuning_key, time.time() - t0))
     ref = ctx.pi_batch(xs, use_graph=True)
     # both batch slots get their arena and launch graphs before the timed run
     for r in ctx.pi_batches([xs] * max(2, args.warmup)):
@@ -464,7 +519,7 @@ def main():
     terms = ctx.pi_batch(xs, terms=True)
     for x, t in zip(xs, terms):
         t["x"] = x
-    primes = primes_upto(200000)
+    primes = primes_upto(iroot(max(xs), 2) + 1)
     w_lds = algorithmic_lds_bytes(terms, primes)
     seg_avg_s = (sum(seg_ms) / len(seg_ms)) / 1000.0
 
@@ -547,19 +602,6 @@ def main():
             # every x of the stream solved alone (one batch), to check each served value
             uniq = sorted(set(sx))
             alone = dict(zip(uniq, ctx.pi_batch(uniq)))
-            srv = C.Server(ctx, max_x=max(sx))
-            tickets = srv.replay(sx, st_ms)
-            srv.drain()
-            res = srv.poll(len(sx) + 16)
-            stt = srv.stats()
-            srv.close()
-            got = {r["ticket"]: r for r in res}
-            ok = all(r["status"] == 0 for r in res) and len(res) == len(sx)
-            matches = sum(1 for tk, x in zip(tickets, sx)
-                          if tk in got and got[tk]["status"] == 0 and got[tk]["pi"] == alone[x])
-            if matches != len(sx):
-                raise SystemExit("served lambda=%d: %d of %d results differ from the same x solved alone"
-                                 % (lam, len(sx) - matches, len(sx)))
             # the same x list one query at a time (warm tables, no batching)
             pw = None
             if not args.no_plain:
@@ -571,49 +613,84 @@ def main():
                         if wc.pi(x) != alone[x]:
                             raise SystemExit("plain solve of %d differs" % x)
                     pw = len(sx) / (time.time() - t0p)
-            waits = sorted(r["t_admit_ms"] - r["t_arrive_ms"] for r in res)
-            qw = (lambda f: round(waits[min(len(waits) - 1, int(f * len(waits)))], 3)) if waits else \
-                (lambda f: None)
-            served[str(lam)] = {"n": len(sx), "qps": round(stt["qps"], 2),
-                                "p50_ms": round(stt["p50_ms"], 3), "p99_ms": round(stt["p99_ms"], 3),
-                                "mean_ms": round(stt["mean_ms"], 3),
-                                "queue_p50_ms": qw(0.5), "queue_p99_ms": qw(0.99),
-                                "submitted": stt["submitted"], "completed": stt["completed"],
-                                "rejected": stt["rejected"], "batches": stt["batches"],
-                                "max_inflight_bytes": stt["max_inflight_bytes"],
-                                "launches": stt["launches"], "joined": stt["joined"],
-                                "all_ok": ok, "matches_alone": matches,
-                                "plain_warm_qps_same_x": round(pw, 2) if pw else None,
-                                "speedup_vs_plain_warm": round(stt["qps"] / pw, 2) if pw else None}
-            log("served lambda=%d: %s" % (lam, served[str(lam)]))
+            for mode, key in ((C.SERVER_STREAM, str(lam)), (C.SERVER_BATCHES, "%d_batches" % lam)):
+                srv = C.Server(ctx, max_x=max(sx), mode=mode)
+                tickets = srv.replay(sx, st_ms)
+                srv.drain()
+                res = srv.poll(len(sx) + 16)
+                stt = srv.stats()
+                srv.close()
+                got = {r["ticket"]: r for r in res}
+                ok = all(r["status"] == 0 for r in res) and len(res) == len(sx)
+                matches = sum(1 for tk, x in zip(tickets, sx)
+                              if tk in got and got[tk]["status"] == 0 and got[tk]["pi"] == alone[x])
+                if matches != len(sx):
+                    raise SystemExit("served lambda=%d mode %d: %d of %d results differ from the same x "
+                                     "solved alone" % (lam, mode, len(sx) - matches, len(sx)))
+                waits = sorted(r["t_admit_ms"] - r["t_arrive_ms"] for r in res)
+                qw = (lambda f: round(waits[min(len(waits) - 1, int(f * len(waits)))], 3)) if waits else \
+                    (lambda f: None)
+                served[key] = {"mode": "stream" if mode == C.SERVER_STREAM else "batches",
+                               "lambda": lam, "n": len(sx), "qps": round(stt["qps"], 2),
+                               "p50_ms": round(stt["p50_ms"], 3), "p99_ms": round(stt["p99_ms"], 3),
+                               "mean_ms": round(stt["mean_ms"], 3),
+                               "queue_p50_ms": qw(0.5), "queue_p99_ms": qw(0.99),
+                               "submitted": stt["submitted"], "completed": stt["completed"],
+                               "rejected": stt["rejected"], "batches": stt["batches"],
+                               "max_inflight_bytes": stt["max_inflight_bytes"],
+                               "launches": stt["launches"], "joined": stt["joined"],
+                               "all_ok": ok, "matches_alone": matches,
+                               "plain_warm_qps_same_x": round(pw, 2) if pw else None,
+                               "speedup_vs_plain_warm": round(stt["qps"] / pw, 2) if pw else None}
+                log("served lambda=%d: %s" % (lam, served[key]))
         out["served"] = served
 
-    # ---- roofline of the persistent segment kernel (the LDS bound of the
-    # staged sieve shared by each query group; HBM and the integer VALU rate
-    # beside it)
-    lds_peak, int_peak = ctx.probe_peaks()
-    ach = w_lds / seg_avg_s / 1e9
-    traffic = profiled_traffic("batch")
-    out["roofline"] = {"bound": "lds", "achieved": round(ach, 1), "peak": round(lds_peak / 1e9, 1),
-                       "unit": "GB/s", "frac": round(ach / (lds_peak / 1e9), 4),
-                       "traffic": traffic,
-                       "kernel": "k_segments (persistent: hard-leaf + P2 segment tiles, easy-leaf and "
-                                 "P2-lookup tiles)",
-                       "kernel_ms": round(seg_avg_s * 1000, 3),
-                       "algorithmic_bytes": w_lds,
-                       "peak_source": "measured on this device by cpx_probe_peaks (LDS 16-byte reads, all CUs)",
-                       "components": {
-                           "hbm": {"achieved": round(traffic / seg_avg_s / 1e9, 1) if traffic else None,
-                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": round(traffic / seg_avg_s / 1e9 / HBM_PEAK_GBS, 5) if traffic else None},
-                           "int32_valu": {"peak": round(int_peak / 1e12, 2), "unit": "Tops/s (lane ops)",
-                                          "issue_frac": profiled_valu("batch"),
-                                          "issue_frac_source": "SQ_INSTS_VALU / kernel time over the "
-                                                               "measured issue peak, profiles/k_segments_batch_pmc.md"}}}
+    # ---- roofline of the persistent segment kernel.  The counters show VALU
+    # issue as its busiest resource (LDS and HBM far below their peaks), so
+    # the bound is VALU: instructions per launch (rocprofv3 SQ_INSTS_VALU on
+    # the same workload and tuning, committed) over the live kernel time.
+    # W_LDS and W_HBM are models, reported beside the counted bytes.
+    lds_probe, int_probe = ctx.probe_peaks()
+    valu_probe_g = int_probe / 64.0 / 1e9              # lane ops -> wave64 instructions
+    pc = profiled_counters("batch")
+    same = pc.get("tuning") == tuning_key
+    instr = pc.get("valu_instr_per_launch") if same else None
+    ach = instr / seg_avg_s / 1e9 if instr else None
+    traffic = int(pc["hbm_bytes_per_launch"]) if same and "hbm_bytes_per_launch" in pc else None
+    lds_counted = pc.get("lds_bytes_per_launch") if same else None
+    w_hbm, hbm_parts = hbm_model_bytes(terms, primes, 32 * shape[0] * shape[1])
+    out["roofline"] = {
+        "bound": "valu", "achieved": round(ach, 1) if ach else None, "peak": round(VALU_PEAK_G, 1),
+        "unit": "G wave64-instr/s", "frac": round(ach / VALU_PEAK_G, 4) if ach else None,
+        "traffic": traffic,
+        "kernel": "k_segments (persistent: hard-leaf + P2 segment tiles, easy-leaf and P2-lookup tiles)",
+        "kernel_ms": round(seg_avg_s * 1000, 3),
+        "valu_instr_per_launch": instr,
+        "peak_source": "2 wave64 32-bit VALU instructions per CU-clock (MI355X FP32 vector peak 157.3 TFLOP/s "
+                       "= 256 CUs x 2.4 GHz x 128 FMA lanes)",
+        "probe_peak": round(valu_probe_g, 1),
+        "frac_vs_probe": round(ach / valu_probe_g, 4) if ach else None,
+        "probe_source": "cpx_probe_peaks: 8 independent add/shift/xor chains per lane, 24 VALU per iteration "
+                        "(SQ_INSTS_VALU matches; profiles/peaks.md), measured live",
+        "counters_source": "profiles/k_segments_batch_pmc.md at tuning %s%s"
+                           % (pc.get("tuning"), "" if same else " (differs from this run: not used)"),
+        "components": {
+            "lds": {"counted_bytes": lds_counted,
+                    "counted_GBs": round(lds_counted / seg_avg_s / 1e9, 1) if lds_counted else None,
+                    "model_bytes": w_lds,
+                    "model": "W_LDS = B_h*ceil(z/2)/8 + 8*sum ceil((z/2)/p_b) per query group (one "
+                             "popcount pass per stage + one RMW per cross-off); the kernel does less "
+                             "(no per-stage pass), so W_LDS over time is not a bandwidth",
+                    "peak_GBs": round(LDS_PEAK_GBS, 1), "probe_GBs": round(lds_probe / 1e9, 1),
+                    "frac_counted": round(lds_counted / seg_avg_s / 1e9 / LDS_PEAK_GBS, 4) if lds_counted else None},
+            "hbm": {"counted_bytes": traffic,
+                    "counted_GBs": round(traffic / seg_avg_s / 1e9, 1) if traffic else None,
+                    "model_bytes": w_hbm, "model_parts": hbm_parts,
+                    "peak_GBs": HBM_PEAK_GBS,
+                    "frac_counted": round(traffic / seg_avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}}}
     if "single_1e16" in out:
         s16 = out["single_1e16"]
-        s16["lds_roofline_frac"] = round(s16["lds_algorithmic_GB"] / (s16["segments_ms"] / 1000.0)
-                                         / (lds_peak / 1e9), 4)
+        s16["lds_model_GBs"] = round(s16["lds_algorithmic_GB"] / (s16["segments_ms"] / 1000.0), 1)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(xs, full=args.cpu_full)

@@ -58,7 +58,12 @@ class CpxBatchStats(ctypes.Structure):
 
 class CpxServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", U64), ("max_batch", ctypes.c_uint32),
-                ("max_wait_ms", ctypes.c_double), ("slots", ctypes.c_uint32), ("max_x", CpxU128)]
+                ("max_wait_ms", ctypes.c_double), ("slots", ctypes.c_uint32), ("max_x", CpxU128),
+                ("mode", ctypes.c_uint32)]
+
+
+SERVER_STREAM = 0    # cohorts join one running streaming launch (continuous batching)
+SERVER_BATCHES = 1   # each admitted FIFO prefix runs as a batch launch of its own
 
 
 class CpxResult(ctypes.Structure):
@@ -671,12 +676,15 @@ class Phi:
 
 
 class Server:
-    """Continuous-batching pi(x) server: requests join the running streaming
-    launch as cohorts, admitted by the HBM bytes of their planned state."""
+    """pi(x) server.  mode SERVER_STREAM: continuous batching, requests join
+    the running streaming launch as cohorts, admitted by the HBM bytes of
+    their planned state; SERVER_BATCHES: each admitted FIFO prefix runs as a
+    batch launch of its own while later arrivals queue."""
 
-    def __init__(self, ctx, hbm_budget=0, max_batch=256, max_wait_ms=2.0, slots=4, max_x=0):
+    def __init__(self, ctx, hbm_budget=0, max_batch=256, max_wait_ms=2.0, slots=4, max_x=0,
+                 mode=SERVER_STREAM):
         self.ctx = ctx
-        o = CpxServerOpts(hbm_budget, max_batch, max_wait_ms, slots, u128(max_x))
+        o = CpxServerOpts(hbm_budget, max_batch, max_wait_ms, slots, u128(max_x), mode)
         h = ctypes.c_void_p()
         ctx._chk(ctx._L.cpx_server_create(ctx.h, ctypes.byref(o), ctypes.byref(h)), "Server")
         self.h = h

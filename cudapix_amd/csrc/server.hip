@@ -1,4 +1,5 @@
-// server.hip -- continuous-batching pi(x) server over the streaming launch.
+// server.hip -- pi(x) server: continuous batching over the streaming launch,
+// or (CPX_SERVER_BATCHES) one batch launch per admitted FIFO prefix.
 //
 // A dispatcher thread owns an executor context (tables resident across
 // requests; the caller's tuning snapshotted at creation) and at most one
@@ -454,6 +455,70 @@ static void dispatcher(cpx_server *s)
   close_launch(s, lk);
 }
 
+// Batch mode: the dispatcher takes the longest FIFO prefix (at most
+// max_batch requests whose planned bytes fit the budget) and runs it as one
+// batch launch on the executor context; requests that arrive meanwhile
+// queue for the next.  Nothing stays resident on the device between batches.
+static void batch_dispatcher(cpx_server *s)
+{
+  std::vector<Req> grp;
+  std::vector<cpx_u128> xs, out;
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+    if (s->pending.empty()) {
+      if (s->stop) break;
+      continue;
+    }
+    grp.clear();
+    uint64_t gbytes = 0;
+    while (!s->pending.empty() && grp.size() < s->opts.max_batch) {
+      const Req r = s->pending.front();
+      if (r.cancelled) {                 // its result was reported by the cancel
+        s->pending.pop_front();
+        continue;
+      }
+      HostQuery h;
+      int rc = make_host_query(s->lane, ((u128)r.x.hi << 64) | r.x.lo, 0, 0, &h);
+      const uint64_t b = rc == CPX_OK ? plan_bytes_estimate(s->lane, h) : 0;
+      if (rc == CPX_OK && b > s->budget) rc = CPX_ENOMEM;
+      if (rc != CPX_OK) {
+        s->pending.pop_front();
+        const double now = srv_now_ms();
+        finish(s, r, {0, 0}, rc, now, now);
+        continue;
+      }
+      if (!grp.empty() && gbytes + b > s->budget) break;
+      gbytes += b;
+      grp.push_back(r);
+      s->state[r.ticket] = 1;
+      s->pending.pop_front();
+    }
+    s->cv_done.notify_all();
+    if (grp.empty()) continue;
+    s->planning++;                       // drain() waits for it
+    s->batches++;
+    const uint64_t no = ++s->launch_no;
+    s->inflight_bytes = gbytes;
+    s->max_inflight = std::max(s->max_inflight, gbytes);
+    lk.unlock();
+    const double t_admit = srv_now_ms();
+    xs.resize(grp.size());
+    out.assign(grp.size(), cpx_u128{0, 0});
+    for (size_t i = 0; i < grp.size(); i++) xs[i] = grp[i].x;
+    // P2 by lookups only where the resident pi table reaches: growing it
+    // would stall the queue (tables for max_x are built at creation)
+    cpx_batch_opts bo{1, 0, 1};
+    const int rc = cpx_pi_batch(s->lane, xs.data(), xs.size(), out.data(), &bo, nullptr);
+    const double t_done = srv_now_ms();
+    lk.lock();
+    s->inflight_bytes = 0;
+    for (size_t i = 0; i < grp.size(); i++) finish(s, grp[i], out[i], rc, t_admit, t_done, no, no);
+    s->planning--;
+    s->cv_done.notify_all();
+  }
+}
+
 extern "C" {
 
 int cpx_server_create(cpx_ctx *c, const cpx_server_opts *o, cpx_server **out)
@@ -464,6 +529,10 @@ int cpx_server_create(cpx_ctx *c, const cpx_server_opts *o, cpx_server **out)
   if (o) s->opts = *o;
   if (s->opts.max_batch == 0) s->opts.max_batch = 256;
   if (s->opts.max_wait_ms <= 0) s->opts.max_wait_ms = 2.0;
+  if (s->opts.mode != CPX_SERVER_STREAM && s->opts.mode != CPX_SERVER_BATCHES) {
+    delete s;
+    return CPX_EINVAL;
+  }
   s->nslots = s->opts.slots ? std::min<uint32_t>(s->opts.slots, CPX_STREAM_SLOTS) : 4;
   s->budget = s->opts.hbm_budget ? s->opts.hbm_budget : (64ull << 30);
   auto fail = [&](int rc) {
@@ -482,7 +551,7 @@ int cpx_server_create(cpx_ctx *c, const cpx_server_opts *o, cpx_server **out)
     std::lock_guard<std::recursive_mutex> g(c->mu);
     lo = c->opts;
     lo.fresh_tables = 0;
-    lo.use_graphs = 0;
+    lo.use_graphs = s->opts.mode == CPX_SERVER_BATCHES ? 1 : 0;
     lo.hbm_budget = 0;
     int rc = cpx_ctx_create(&lo, &s->lane);
     if (rc) return fail(rc);
@@ -491,6 +560,21 @@ int cpx_server_create(cpx_ctx *c, const cpx_server_opts *o, cpx_server **out)
     s->lane->alpha_scale = c->alpha_scale;
   }
   cpx_ctx *lc = s->lane;
+  if (s->opts.mode == CPX_SERVER_BATCHES) {
+    // tables (with P2 lookups) for the largest x announced
+    const u128 mx = ((u128)s->opts.max_x.hi << 64) | s->opts.max_x.lo;
+    if (mx >= CPX_SMALL_X) {
+      HostQuery h;
+      int rc = make_host_query(lc, mx, 0, 0, &h);
+      std::vector<HostQuery> v(1, h);
+      if (!rc) rc = stream_grow_tables(lc, v, true);
+      if (rc) return fail(rc);
+    }
+    s->t0 = srv_now_ms();
+    s->th = std::thread(batch_dispatcher, s);
+    *out = s;
+    return CPX_OK;
+  }
   if (!stream_shape_supported(lc->shape)) lc->shape = SegShape{512, 32};
   s->shape = lc->shape;
   {

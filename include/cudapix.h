@@ -274,8 +274,10 @@ int cpx_set_tuning(cpx_ctx *ctx, uint32_t threads, uint32_t kwords, double alpha
  * kernel that keeps taking work while the server publishes new cohorts of
  * admitted requests into it.  Requests that arrive while others run join the
  * running launch as soon as a cohort slot and the HBM budget allow; their
- * tiles run beside the earlier cohorts' tail.  The reference has no serving
- * path (src/main.cpp:23-88 runs one term per process). */
+ * tiles run beside the earlier cohorts' tail.  In CPX_SERVER_BATCHES mode the
+ * server instead runs each admitted FIFO prefix as a batch launch of its own.
+ * The reference has no serving path (src/main.cpp:23-88 runs one term per
+ * process). */
 typedef struct {
   uint64_t hbm_budget;     /* device bytes for the cohort arena (0: 64 GiB): a cohort is admitted
                               when its planned state fits the space left in the running launch;
@@ -288,7 +290,14 @@ typedef struct {
   uint32_t slots;          /* cohorts in flight at once, 1..8 (default 4) */
   cpx_u128 max_x;          /* build the resident tables for x up to this at creation (0: grow on
                               demand, at a launch boundary) */
+  uint32_t mode;           /* CPX_SERVER_STREAM (default): cohorts join one running streaming
+                              launch; CPX_SERVER_BATCHES: each admitted FIFO prefix (at most
+                              max_batch requests, admitted by their planned bytes) runs as a batch
+                              launch of its own while later arrivals queue for the next one */
 } cpx_server_opts;
+
+#define CPX_SERVER_STREAM 0u
+#define CPX_SERVER_BATCHES 1u
 
 typedef struct {
   uint64_t ticket;

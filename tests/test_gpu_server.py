@@ -1,8 +1,8 @@
-"""Served queries through the continuous-batching server (tolerance 0): each
+"""Served queries through the server (tolerance 0): each
 result equals the same x solved alone; requests submitted while a launch runs
 join that launch and their tiles run beside the earlier cohort's on the
 device; a slice of the served workload (x to 1e17 at 2000 requests/s) is
-value-checked; admission rejects a request larger than a slot's share of the
+value-checked, in both server modes; admission rejects a request larger than a slot's share of the
 HBM budget and cancels queued ones; the server serves with the tuning of the
 context it was created from."""
 import json
@@ -101,6 +101,33 @@ def test_served_stream_slice_c4(ctx):
         r = res[tk]
         assert r["status"] == 0, r
         assert r["pi"] == ctx.pi(x), x
+
+
+def test_batch_mode_server_c4_slice(ctx):
+    """CPX_SERVER_BATCHES: the same 300-request slice (x to 1e17) plus tiny x,
+    each value equal to the same x solved alone, every request served by one
+    of the server's own batch launches; with a one-byte budget every request
+    is rejected with CPX_ENOMEM."""
+    import cudapix_amd as C
+    xs, ts = c4(2000)
+    xs, ts = xs[:300] + [5, 16383], ts[:300] + [ts[299] + 1.0, ts[299] + 2.0]
+    srv = C.Server(ctx, max_x=max(xs), mode=C.SERVER_BATCHES)
+    tickets, res = served(srv, xs, ts)
+    st = srv.stats()
+    srv.close()
+    print("batch-mode slice", st, flush=True)
+    for tk, x in zip(tickets, xs):
+        r = res[tk]
+        assert r["status"] == 0, r
+        assert r["pi"] == ctx.pi(x), x
+        assert 1 <= r["launch"] <= st["launches"]
+    assert st["launches"] == st["batches"] >= 2 and st["joined"] == 0
+    assert st["completed"] == len(xs)
+    srv = C.Server(ctx, hbm_budget=1, mode=C.SERVER_BATCHES)
+    tickets, res = served(srv, [10 ** 9, 10 ** 12], [0.0, 1.0])
+    st = srv.stats()
+    srv.close()
+    assert all(res[t]["status"] == -2 for t in tickets) and st["rejected"] == 2
 
 
 def test_server_budget_rejects_and_cancels(ctx):
