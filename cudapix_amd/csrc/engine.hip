@@ -70,22 +70,28 @@ static int dbuf_reserve(cpx_ctx *c, DBuf<T> &b, size_t n, uint64_t *acct = nullp
   return CPX_OK;
 }
 
+// Frees b and takes its bytes off the account they were charged to.
 template <typename T>
-static void dbuf_free(DBuf<T> &b)
+static void dbuf_free(DBuf<T> &b, uint64_t *acct = nullptr)
 {
-  if (b.p) (void)hipFree(b.p);
+  if (b.p) {
+    (void)hipFree(b.p);
+    if (acct) *acct -= b.bytes();
+  }
   b.p = nullptr;
   b.cap = 0;
 }
 
+// Frees the size-dependent tables (the bootstrap tables and the wheels stay,
+// and stay counted in table_bytes).
 void free_tables(cpx_ctx *c)
 {
   if (c->stream) (void)hipStreamSynchronize(c->stream);   // no queued batch reads them any more
-  dbuf_free(c->pitab); dbuf_free(c->primes); dbuf_free(c->prinv); dbuf_free(c->mlpf);
-  dbuf_free(c->scan_tot); dbuf_free(c->scan_base);
+  uint64_t *a = &c->table_bytes;
+  dbuf_free(c->pitab, a); dbuf_free(c->primes, a); dbuf_free(c->prinv, a); dbuf_free(c->mlpf, a);
+  dbuf_free(c->scan_tot, a); dbuf_free(c->scan_base, a);
   c->pitab_nent = c->pitab_limit = c->prime_limit = c->nprimes = c->mlpf_limit = 0;
   c->h_primes.clear();
-  c->table_bytes = 0;
   c->generation++;
 }
 
@@ -213,8 +219,10 @@ int build_wheels(cpx_ctx *c, uint32_t want)
   if ((rc = dbuf_reserve(c, c->wheel, off, &c->table_bytes))) return rc;
   if ((rc = dbuf_reserve(c, c->winfo, 9, &c->table_bytes))) return rc;
   size_t maxchunks = (period + 4095) / 4096 + 2;
-  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(maxchunks, c->scan_tot.cap)))) return rc;
-  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(maxchunks, c->scan_base.cap)))) return rc;
+  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(maxchunks, c->scan_tot.cap), &c->table_bytes)))
+    return rc;
+  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(maxchunks, c->scan_base.cap), &c->table_bytes)))
+    return rc;
   for (uint32_t k = 1; k <= cmax; k++) {
     HIPCHK(launch_wheel(c->wheel.p + wi[k].off, wi[k].period, k, c->stream));
     HIPCHK(launch_scan_entries(c->wheel.p + wi[k].off, wi[k].period, c->scan_tot.p,
@@ -270,8 +278,9 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
   nent = (nent + 8191) / 8192 * 8192;
   if ((rc = dbuf_reserve(c, c->pitab, nent, &c->table_bytes))) return rc;
   size_t nch = (nent + 4095) / 4096 + 2;
-  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(nch, c->scan_tot.cap)))) return rc;
-  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(nch, c->scan_base.cap)))) return rc;
+  if ((rc = dbuf_reserve(c, c->scan_tot, std::max<size_t>(nch, c->scan_tot.cap), &c->table_bytes))) return rc;
+  if ((rc = dbuf_reserve(c, c->scan_base, std::max<size_t>(nch, c->scan_base.cap), &c->table_bytes)))
+    return rc;
   HIPCHK(launch_pitab_segments(c->pitab.p, nent, c->small_primes.p, c->n_small_primes,
                                c->scan_tot.p, c->scan_base.p, c->stream));
   c->pitab_nent = nent;

@@ -284,9 +284,10 @@ typedef struct {
                               a request whose state alone exceeds the budget is rejected with
                               CPX_ENOMEM */
   uint32_t max_batch;      /* largest cohort (default 256) */
-  double max_wait_ms;      /* a pending request waits at most this long for the device backlog
-                              to drain (so that it joins a larger cohort) before it is
-                              published anyway (default 2) */
+  double max_wait_ms;      /* stream mode: a pending request waits at most this long for the
+                              device backlog to drain (so that it joins a larger cohort) before
+                              it is published anyway (default 2); batch mode starts the next
+                              batch as soon as the previous one ends */
   uint32_t slots;          /* cohorts in flight at once, 1..8 (default 4) */
   cpx_u128 max_x;          /* build the resident tables for x up to this at creation (0: grow on
                               demand, at a launch boundary) */

@@ -190,3 +190,18 @@ def test_div_x_by_y(ctx):
     ys = [3, 17, 2 ** 32 + 15, 10 ** 12, 2 ** 63 + 5]
     assert ctx.divXbyY(x, ys) == [(x // v) % 2 ** 64 for v in ys]
     assert all(x // v < 2 ** 64 for v in ys[1:])
+
+
+def test_device_bytes_keep_the_resident_wheels():
+    """A context with fresh tables rebuilds its size-dependent tables for every
+    query; the bootstrap tables and wheels stay allocated and stay counted, so
+    the reported device bytes are the same after each solve and include the
+    c = 8 wheel (4849845 eight-byte pattern words)."""
+    import cudapix_amd as C
+    with C.Context(device=0, c=8, fresh_tables=True) as fc:
+        assert fc.pi(10 ** 12) == 37607912018
+        b1 = fc.device_bytes()
+        assert fc.pi(10 ** 12) == 37607912018
+        b2 = fc.device_bytes()
+    assert b1 == b2
+    assert b1 >= 8 * 4849845
