@@ -37,6 +37,11 @@ class CpxOpts(ctypes.Structure):
                 ("use_graphs", ctypes.c_uint32)]
 
 
+class CpxSigmaTerms(ctypes.Structure):
+    _fields_ = [("a", CpxI128), ("sigma", CpxI128 * 6), ("s2", CpxI128), ("y", U64),
+                ("exact", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
 class CpxTerms(ctypes.Structure):
     _fields_ = [("pi", CpxU128), ("ordinary", CpxI128), ("trivial", CpxI128),
                 ("easy", CpxI128), ("hard", CpxI128), ("p2", CpxI128),
@@ -89,7 +94,7 @@ EXPORTS = [
     "cpx_p2_u64", "cpx_p2_u128", "cpx_div_x_by_y",
     "cpx_ordinary_u64", "cpx_ordinary_u128",
     "cpx_s2_trivial_u64", "cpx_s2_trivial_u128",
-    "cpx_s3_u64", "cpx_s3_u128", "cpx_v_u64", "cpx_v_u128", "cpx_v_split_u64",
+    "cpx_s3_u64", "cpx_s3_u128", "cpx_v_u64", "cpx_v_u128", "cpx_v_split_u64", "cpx_s2_sigma_u128",
     "cpx_count_primes", "cpx_get_device_primes", "cpx_get_d_pitable",
     "cpx_pitable_create", "cpx_pitable_destroy", "cpx_pitable_current",
     "cpx_pitable_next_up", "cpx_pitable_next_down", "cpx_pitable_info",
@@ -152,6 +157,7 @@ def lib():
         "cpx_v_u64": (ctypes.c_int, [vp, U64, U64, P(ctypes.c_int64)]),
         "cpx_v_u128": (ctypes.c_int, [vp, CpxU128, U64, P(CpxI128)]),
         "cpx_v_split_u64": (ctypes.c_int, [vp, U64, U64, P(ctypes.c_int64), P(ctypes.c_int64)]),
+        "cpx_s2_sigma_u128": (ctypes.c_int, [vp, CpxU128, U64, P(CpxSigmaTerms)]),
         "cpx_count_primes": (ctypes.c_int, [vp, U64, U64, P(U64)]),
         "cpx_get_device_primes": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
         "cpx_get_device_primes32": (ctypes.c_int, [vp, U64, U64, P(vp), P(ctypes.c_size_t)]),
@@ -324,6 +330,17 @@ class Context:
         self._chk(self._L.cpx_v_split_u64(self.h, x, y, ctypes.byref(a), ctypes.byref(b)),
                   "V_split")
         return a.value, b.value
+
+    def s2_sigma(self, x, y=0):
+        """The reference's second form of the easy leaves (S2.cu:30-70, sigma.cu):
+        dict of A, s1 .. s6, S2 = A + s2 + s3 + s4 + 2 s5 + s6, the y used, and
+        exact (S2 equals V: no prime in (x^(1/4), x/y^2])."""
+        o = CpxSigmaTerms()
+        self._chk(self._L.cpx_s2_sigma_u128(self.h, u128(x), y, ctypes.byref(o)), "s2_sigma")
+        d = {"A": from_i128(o.a), "S2": from_i128(o.s2), "y": o.y, "exact": bool(o.exact)}
+        for i in range(6):
+            d["s%d" % (i + 1)] = from_i128(o.sigma[i])
+        return d
 
     def V_a(self, x, y):
         """V over x^(1/4) < p <= sqrt(x/y) (include/V.cuh:7)."""

@@ -309,6 +309,70 @@ int cpx_v_split_u64(cpx_ctx *c, uint64_t x, uint64_t y, int64_t *va, int64_t *vb
   return CPX_OK;
 }
 
+// pi(v) from the resident table (v <= pitab_limit), read on the host.
+static int pi_read(cpx_ctx *c, uint64_t v, uint64_t *out)
+{
+  PiEntry e;
+  HIPCHK(hipMemcpyAsync(&e, c->pitab.p + (v >> 6), sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const uint32_t k = (uint32_t)(((v & 63) + 1) >> 1);
+  *out = (uint64_t)(v >= 2) + e.count + (uint64_t)__builtin_popcount(e.bits & (uint32_t)((1ull << k) - 1));
+  return CPX_OK;
+}
+
+int cpx_s2_sigma_u128(cpx_ctx *c, cpx_u128 xx, uint64_t y, cpx_sigma_terms *out)
+{
+  CTX_GUARD(c);
+  if (!out) return CPX_EINVAL;
+  const u128 x = to_u128(xx);
+  HostQuery h;
+  int rc = make_host_query(c, x, y, 0, &h);
+  if (rc) return rc;
+  if (h.small) return ctx_fail(c, CPX_ERANGE, "x below 2^14 has no easy leaves to partition");
+  y = h.y;
+  const uint64_t q4 = h.qrtx, c3 = h.cbrtx;
+  if ((x / q4) >> 64) return ctx_fail(c, CPX_ERANGE, "x / x^(1/4) must fit 64 bits");
+  const uint64_t sz = cpx_isqrt128(x / y);
+  // the primes to sqrt(x/p) for the smallest p above x^(1/4), pi(n) to sqrt(x)
+  const uint64_t P = std::max<uint64_t>(c3, cpx_isqrt128(x / q4)) + 1;
+  if ((rc = ensure_tables(c, std::max<uint64_t>(h.sqrtx, P), P, 1))) return rc;
+  uint64_t a, b, cc, d;
+  if ((rc = pi_read(c, y, &a)) || (rc = pi_read(c, c3, &b)) || (rc = pi_read(c, q4, &cc)) ||
+      (rc = pi_read(c, sz, &d)))
+    return rc;
+  const uint64_t j0 = cc, np = b > cc ? b - cc : 0;   // primes[j] for j in [pi(q4), pi(c3))
+  i128 A = 0, s4 = 0, s5 = 0, s6 = 0;
+  if (np) {
+    i128 *part = nullptr;
+    HIPCHK(hipMalloc((void **)&part, np * 4 * sizeof(i128)));
+    std::vector<i128> hp(np * 4);
+    hipError_t e = launch_sigma(x, y, sz, c->primes.p, c->pitab.p, j0, np, part, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hp.data(), part, np * 4 * sizeof(i128), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return hip_fail(c, e, "sigma partition");
+    for (uint64_t i = 0; i < np; i++) {
+      A += hp[4 * i]; s4 += hp[4 * i + 1]; s5 += hp[4 * i + 2]; s6 += hp[4 * i + 3];
+    }
+  }
+  const i128 pa = (i128)a, pb = (i128)b, pc = (i128)cc, pd = (i128)d;
+  const i128 sg[6] = {(pa - pb) * (pa - pb - 1) / 2,
+                      pa * (pc * (pc - 3) / 2 - pd * (pd - 3) / 2),
+                      pb * (pb - 1) * (2 * pb - 1) / 6 - pb - pc * (pc - 1) * (2 * pc - 1) / 6 + pc,
+                      pa * s4, s5, s6};
+  auto put = [](i128 v) { return cpx_i128{(uint64_t)v, (uint64_t)((u128)v >> 64)}; };
+  out->a = put(A);
+  for (int i = 0; i < 6; i++) out->sigma[i] = put(sg[i]);
+  out->s2 = put(A + sg[1] + sg[2] + sg[3] + 2 * sg[4] + sg[5]);
+  out->y = y;
+  // exact unless a prime lies in (x^(1/4), x / y^2]: the first prime above x^(1/4)
+  const u128 lim = x / ((u128)y * y);
+  out->exact = !(np && (u128)c->h_primes[j0] <= lim);
+  out->pad = 0;
+  return CPX_OK;
+}
+
 int cpx_div_x_by_y(cpx_ctx *c, cpx_u128 x, uint64_t *d_y, size_t len)
 {
   CTX_GUARD(c);

@@ -217,6 +217,8 @@ hipError_t launch_div_x_by_y(uint64_t x_lo, uint64_t x_hi, uint64_t *d_y, size_t
 
 // terms.hip
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_sigma(u128 x, uint64_t y, uint64_t sz, const uint32_t *primes, const PiEntry *pitab,
+                        uint64_t j0, uint64_t np, i128 *part, hipStream_t s);
 hipError_t launch_setup(BatchDev b, uint32_t maxq, uint32_t max_stage_total, hipStream_t s);
 hipError_t launch_ordinary(BatchDev b, uint32_t grid, hipStream_t s);
 hipError_t launch_trivial(BatchDev b, uint32_t grid, hipStream_t s);

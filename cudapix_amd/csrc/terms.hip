@@ -126,3 +126,61 @@ hipError_t launch_combine(BatchDev b, uint32_t grid, hipStream_t s)
   hipLaunchKernelGGL(k_combine, dim3(grid), dim3(256), 0, s, b);
   return hipGetLastError();
 }
+
+// ------------------------------------------------------------ sigma form
+// The reference's second form of the easy leaves (src/Deleglise-Rivat/S2.cu:30-110
+// with the sums of src/Deleglise-Rivat/sigma.cu:52-112, 159-214): block b takes
+// the prime p = primes[j0 + b] of (x^1/4, x^1/3] and sums over the primes
+// p < q <= sqrt(x/p) the weighted pi(x/pq) of A (weight 2 when x/pq < y),
+// then thread 0 stores the four per-p parts {A_p, pi(x/(p y)) if p <= sqrt z,
+// pi(min(y, x/p^2)) if p > sqrt z, -pi(sqrt(x/p))^2}.  Needs x/p < 2^64,
+// the prime list to sqrt(x/p) and the pi table to sqrt(x).
+static __device__ __forceinline__ uint64_t isqrt64_dev(uint64_t v)
+{
+  uint64_t s = (uint64_t)sqrt((double)v);
+  while ((u128)s * s > v) s--;
+  while ((u128)(s + 1) * (s + 1) <= v) s++;
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sz,
+                                               const uint32_t *__restrict__ primes,
+                                               const PiEntry *__restrict__ pitab, uint64_t j0,
+                                               uint64_t np, i128 *__restrict__ part)
+{
+  __shared__ i128 scratch[256 / CPX_WAVE];
+  const u128 x = ((u128)x_hi << 64) | x_lo;
+  for (uint64_t b = blockIdx.x; b < np; b += gridDim.x) {
+    const uint64_t j = j0 + b;
+    const uint64_t p = primes[j];
+    const uint64_t xp = cpx_div128_to64(x, p);
+    const uint64_t s = isqrt64_dev(xp);
+    const uint64_t kend = pi_lookup(pitab, s);     // primes[k] <= s for k < kend
+    const double xpf = (double)xp;
+    uint64_t acc = 0;
+    for (uint64_t k = j + 1 + threadIdx.x; k < kend; k += blockDim.x) {
+      const uint64_t n = cpx_div_leaf(xp, xpf, primes[k]);   // < sqrt x
+      acc += (n < y ? 2ull : 1ull) * pi_lookup(pitab, n);
+    }
+    const i128 a = block_sum_i128((i128)acc, scratch);
+    if (threadIdx.x == 0) {
+      i128 *o = part + 4 * b;
+      o[0] = a;
+      o[1] = p <= sz ? (i128)pi_lookup(pitab, xp / y) : (i128)0;
+      const uint64_t v = xp / p;
+      o[2] = p > sz ? (i128)pi_lookup(pitab, v < y ? v : y) : (i128)0;
+      const i128 ps = (i128)pi_lookup(pitab, s);
+      o[3] = -ps * ps;
+    }
+  }
+}
+
+hipError_t launch_sigma(u128 x, uint64_t y, uint64_t sz, const uint32_t *primes, const PiEntry *pitab,
+                        uint64_t j0, uint64_t np, i128 *part, hipStream_t s)
+{
+  if (np == 0) return hipSuccess;
+  const uint32_t g = (uint32_t)std::min<uint64_t>(np, 1u << 20);
+  hipLaunchKernelGGL(k_sigma, dim3(g), dim3(256), 0, s, (uint64_t)x, (uint64_t)(x >> 64), y, sz, primes,
+                     pitab, j0, np, part);
+  return hipGetLastError();
+}

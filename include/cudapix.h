@@ -117,6 +117,33 @@ int cpx_v_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, int64_t *out);
 int cpx_v_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, cpx_i128 *out);
 /* int64_t V_a / V_b(x, pi, p, q)                  include/V.cuh:7-8: V split at p = sqrt(x/y) */
 int cpx_v_split_u64(cpx_ctx *ctx, uint64_t x, uint64_t y, int64_t *v_a, int64_t *v_b);
+/* The reference's second form of the easy leaves: deleglise_rivat64::S2()
+ * (src/Deleglise-Rivat/S2.cu:30-70) = A + sigma2 + sigma3 + sigma4 + 2 sigma5
+ * + sigma6, with A the weighted sum of S2_a (S2.cu:72-110) and sigma1..sigma6
+ * of src/Deleglise-Rivat/sigma.cu:25-112 (the sums taken with the offsets of
+ * their 128-bit twins, sigma.cu:118-214, and sigma5 capped at pi(y)):
+ *   A      = sum_{x^1/4 < p <= x^1/3} sum_{p < q <= sqrt(x/p)} w pi(x/pq),
+ *            w = 2 if x/pq < y else 1
+ *   sigma1 = C(pi(y) - pi(x^1/3), 2)   (part of the trivial leaves, not of S2)
+ *   sigma2 = pi(y) (pi4 (pi4 - 3) / 2 - piz (piz - 3) / 2)
+ *   sigma3 = pi3 (pi3 - 1)(2 pi3 - 1) / 6 - pi3 - pi4 (pi4 - 1)(2 pi4 - 1) / 6 + pi4
+ *   sigma4 = pi(y) sum_{x^1/4 < p <= sqrt z} pi(x / (p y))
+ *   sigma5 = sum_{sqrt z < p <= x^1/3} pi(min(y, x / p^2))
+ *   sigma6 = - sum_{x^1/4 < p <= x^1/3} pi(sqrt(x / p))^2
+ * (pi3 = pi(x^1/3), pi4 = pi(x^1/4), piz = pi(sqrt z), z = x / y, every root
+ * an exact integer root).  s2 equals the direct V (cpx_v_*) exactly when no
+ * prime lies in (x^1/4, x / y^2] (exact = 1); otherwise A also counts the
+ * pairs y < q <= sqrt(x/p) and s2 exceeds V.  y = 0 picks y from alpha;
+ * x / x^(1/4) must fit 64 bits (CPX_ERANGE otherwise). */
+typedef struct {
+  cpx_i128 a;
+  cpx_i128 sigma[6];       /* sigma1 .. sigma6 */
+  cpx_i128 s2;             /* A + sigma2 + sigma3 + sigma4 + 2 sigma5 + sigma6 */
+  uint64_t y;              /* y used */
+  uint32_t exact;          /* 1: s2 equals V for this x and y */
+  uint32_t pad;
+} cpx_sigma_terms;
+int cpx_s2_sigma_u128(cpx_ctx *ctx, cpx_u128 x, uint64_t y, cpx_sigma_terms *out);
 
 /* ---- primes, pi tables, phi (CUDASieve / pitable.cuh / phi.cuh) --------- */
 /* CudaSieve::countPrimes(bottom, top): primes in [lo, hi) */

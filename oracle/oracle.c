@@ -586,6 +586,72 @@ void orc_hard(uint64_t xhi, uint64_t xlo, uint64_t y, uint32_t c, int threads, u
   free(part); free(cnt); free(bm); free(mu); free(lpf); free(pr);
 }
 
+/* ------------------------------------------------ sigma partition of S2 */
+
+/* The reference's second form of the easy leaves (src/Deleglise-Rivat/S2.cu:30-70
+ * with the sigma terms of src/Deleglise-Rivat/sigma.cu:25-112, taking the
+ * offsets of their 128-bit twins, sigma.cu:118-214):
+ *   A  = sum_{x^1/4 < p <= x^1/3} sum_{p < q <= sqrt(x/p)} chi(x/pq) pi(x/pq),
+ *        chi(n) = 2 if n < y else 1                              (S2.cu:72-110)
+ *   s1 = C(pi(y) - pi(x^1/3), 2)
+ *   s2 = pi(y) [pi(x^1/4)(pi(x^1/4) - 3)/2 - pi(sqrt z)(pi(sqrt z) - 3)/2]
+ *   s3 = pi3(pi3 - 1)(2 pi3 - 1)/6 - pi3 - pi4(pi4 - 1)(2 pi4 - 1)/6 + pi4,
+ *        pi3 = pi(x^1/3), pi4 = pi(x^1/4)
+ *   s4 = pi(y) sum_{x^1/4 < p <= sqrt z} pi(x/(p y))
+ *   s5 = sum_{sqrt z < p <= x^1/3} pi(min(y, x/p^2))
+ *   s6 = - sum_{x^1/4 < p <= x^1/3} pi(floor(sqrt(x/p)))^2
+ *   S2 = A + s2 + s3 + s4 + 2 s5 + s6  (s1 belongs to the trivial leaves).
+ * S2 equals the direct V exactly when no prime lies in (x^1/4, x/y^2]; with
+ * such primes A also counts pairs y < q <= sqrt(x/p).  out (two 64-bit words
+ * each, two's complement): A, s1 .. s6, S2.  Needs x/x^(1/4) < 2^64. */
+void orc_sigma(uint64_t xhi, uint64_t xlo, uint64_t y, int threads, uint64_t out[16])
+{
+  u128 x = mk128(xhi, xlo);
+  uint64_t q4 = orc_iroot4(xhi, xlo), c3 = orc_icbrt(xhi, xlo), sq = orc_isqrt(xhi, xlo);
+  u128 z128 = x / y;
+  uint64_t sz = orc_isqrt((uint64_t)(z128 >> 64), (uint64_t)z128);
+  size_t np; uint32_t *pr = primes_upto(sq, &np);           /* pi(n) for every n used is <= sqrt x */
+  const i128 a = (i128)count_le(pr, np, y), b = (i128)count_le(pr, np, c3);
+  const i128 c = (i128)count_le(pr, np, q4), d = (i128)count_le(pr, np, sz);
+  const uint64_t j0 = count_le(pr, np, q4), j1 = count_le(pr, np, c3);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  i128 A = 0, s4 = 0, s5 = 0, s6 = 0;
+#pragma omp parallel
+  {
+    i128 la = 0, l4 = 0, l5 = 0, l6 = 0;
+#pragma omp for schedule(dynamic, 16)
+    for (uint64_t j = j0; j < j1; j++) {
+      const uint64_t p = pr[j];
+      const uint64_t xp = (uint64_t)(x / p);
+      const uint64_t s = orc_isqrt(0, xp);
+      for (uint64_t k = j + 1; k < np && pr[k] <= s; k++) {
+        const uint64_t n = xp / pr[k];
+        la += (i128)(n < y ? 2 : 1) * (i128)count_le(pr, np, n);
+      }
+      if (p <= sz) l4 += (i128)count_le(pr, np, xp / y);
+      else {
+        const uint64_t v = xp / p;
+        l5 += (i128)count_le(pr, np, v < y ? v : y);
+      }
+      const i128 ps = (i128)count_le(pr, np, s);
+      l6 -= ps * ps;
+    }
+#pragma omp critical
+    { A += la; s4 += l4; s5 += l5; s6 += l6; }
+  }
+  s4 *= a;
+  const i128 s1 = (a - b) * (a - b - 1) / 2;
+  const i128 s2 = a * (c * (c - 3) / 2 - d * (d - 3) / 2);
+  const i128 s3 = b * (b - 1) * (2 * b - 1) / 6 - b - c * (c - 1) * (2 * c - 1) / 6 + c;
+  const i128 S2 = A + s2 + s3 + s4 + 2 * s5 + s6;
+  put128((u128)A, out + 0); put128((u128)s1, out + 2); put128((u128)s2, out + 4);
+  put128((u128)s3, out + 6); put128((u128)s4, out + 8); put128((u128)s5, out + 10);
+  put128((u128)s6, out + 12); put128((u128)S2, out + 14);
+  free(pr);
+}
+
 /* ------------------------------------------------------------ assembly */
 
 /* terms: [0..1] S0, [2..3] trivial, [4..5] V, [6..7] S3, [8..9] P2, [10..11] pi(y),

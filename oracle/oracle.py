@@ -72,6 +72,8 @@ def _load(path):
     L.orc_easy_part.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, out2]
     L.orc_hard.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
     L.orc_pi_dr.argtypes = [U64, U64, U64, ctypes.c_uint32, ctypes.c_int, out2]
+    L.orc_sigma.argtypes = [U64, U64, U64, ctypes.c_int, out2]
+    L.orc_sigma.restype = None
     for n in ("orc_p2", "orc_ordinary", "orc_trivial", "orc_easy", "orc_easy_part", "orc_hard",
               "orc_pi_dr"):
         getattr(L, n).restype = None
@@ -184,6 +186,15 @@ def V_b(x, y, c=6, threads=0):
 
 def S3(x, y, c=6, threads=0):
     return _term(lib().orc_hard, *_split(x), y, c, threads)
+
+
+def sigma(x, y, threads=0):
+    """The sigma partition of the easy leaves (oracle.c orc_sigma): dict of
+    A, s1 .. s6 and S2 = A + s2 + s3 + s4 + 2 s5 + s6."""
+    out = (U64 * 16)()
+    lib().orc_sigma(*_split(x), y, threads, out)
+    keys = ("A", "s1", "s2", "s3", "s4", "s5", "s6", "S2")
+    return {k: _s128(out[2 * i], out[2 * i + 1]) for i, k in enumerate(keys)}
 
 
 def pi_dr(x, y, c=6, threads=0):

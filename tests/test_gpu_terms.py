@@ -56,3 +56,20 @@ def test_single_term_entry_points(ctx, oracle):
 def test_small_x_table_path(ctx, oracle):
     for x in [0, 1, 2, 3, 4, 10, 100, 1000, 1023, 4096, 10 ** 4, 16383]:
         assert ctx.pi(x) == oracle.count_primes(0, x + 1)
+
+
+@pytest.mark.parametrize("x,alpha", [(10 ** 9, 1.0), (10 ** 10, 3.0), (10 ** 12, 2.0), (10 ** 12, 4.0),
+                                     (2 ** 40 + 15, 6.0), (10 ** 13, 1.5), (10 ** 14, 8.0)])
+def test_s2_sigma_form_matches_oracle(ctx, oracle, x, alpha):
+    """The sigma form of the easy leaves (A and sigma1..sigma6 of the reference's
+    S2.cu / sigma.cu) term by term against the oracle; where it is exact (no
+    prime in (x^1/4, x/y^2]) its sum equals the direct V of the same y."""
+    y = oracle.choose_y(x, alpha)
+    print("case sigma x=%d y=%d" % (x, y), flush=True)
+    g = ctx.s2_sigma(x, y)
+    r = oracle.sigma(x, y)
+    assert g["y"] == y
+    for k in ("A", "s1", "s2", "s3", "s4", "s5", "s6", "S2"):
+        assert g[k] == r[k], k
+    v = ctx.V(x, y)
+    assert (g["S2"] == v) == g["exact"], (g["S2"], v, g["exact"])

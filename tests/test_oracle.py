@@ -150,3 +150,75 @@ def test_query_fixtures_regenerate():
     for s in c4.values():
         assert len(s["x"]) == 2000 and all(10 ** 8 <= x <= 10 ** 17 for x in s["x"])
         assert all(a < b for a, b in zip(s["t_ms"], s["t_ms"][1:]))
+
+
+def _sigma_brute(x, y):
+    """A and sigma2..sigma6 from their definitions with a numpy sieve (for the
+    oracle's orc_sigma), and the direct V."""
+    import numpy as np
+    sq = int(x ** 0.5) + 2
+    s = np.ones(sq + 1, bool)
+    s[:2] = False
+    for i in range(2, int(sq ** 0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    pc = np.cumsum(s)
+    pr = [int(v) for v in np.nonzero(s)[0]]
+
+    def root(v, k):
+        r = int(round(v ** (1.0 / k)))
+        while r ** k > v:
+            r -= 1
+        while (r + 1) ** k <= v:
+            r += 1
+        return r
+
+    def pi(n):
+        return int(pc[n])
+
+    q4, c3, sz = root(x, 4), root(x, 3), root(x // y, 2)
+    P1 = [p for p in pr if q4 < p <= c3]
+    A = sum((2 if x // (p * q) < y else 1) * pi(x // (p * q))
+            for p in P1 for q in pr if p < q <= root(x // p, 2))
+    V = sum(pi(x // (p * q)) - pi(p) + 2 for p in P1 for q in pr if p < q <= min(y, x // (p * p)))
+    a, b, c, d = pi(y), pi(c3), pi(q4), pi(sz)
+    s2 = a * (c * (c - 3) // 2 - d * (d - 3) // 2)
+    s3 = b * (b - 1) * (2 * b - 1) // 6 - b - c * (c - 1) * (2 * c - 1) // 6 + c
+    s4 = a * sum(pi(x // (p * y)) for p in P1 if p <= sz)
+    s5 = sum(pi(min(y, x // (p * p))) for p in P1 if p > sz)
+    s6 = -sum(pi(root(x // p, 2)) ** 2 for p in P1)
+    exact = not any(p * y * y <= x for p in P1)
+    return {"A": A, "s2": s2, "s3": s3, "s4": s4, "s5": s5, "s6": s6}, V, exact
+
+
+def test_sigma_partition_against_definitions(oracle):
+    """orc_sigma against the definitions (numpy sieve), and the identity
+    A + s2 + s3 + s4 + 2 s5 + s6 = V exactly when no prime lies in
+    (x^1/4, x/y^2] -- and not otherwise (A then also counts q > y)."""
+    rnd = random.Random(21)
+    seen = {True: 0, False: 0}
+    for _ in range(24):
+        x = rnd.randrange(10 ** 6, 3 * 10 ** 8)
+        cb = oracle.icbrt(x)
+        y = rnd.randrange(cb, min(oracle.isqrt(x), 5 * cb) + 1)
+        want, V, exact = _sigma_brute(x, y)
+        got = oracle.sigma(x, y)
+        for k, v in want.items():
+            assert got[k] == v, (x, y, k)
+        assert (got["S2"] == V) == exact, (x, y)
+        seen[exact] += 1
+    assert seen[True] and seen[False]
+
+
+def test_sigma_partition_equals_golden_easy_leaves(oracle):
+    # where the partition is exact, it gives the brute-force easy leaves
+    n = 0
+    for case in load("terms.json"):
+        x, y = case["x"], case["y"]
+        if x < 16384:
+            continue
+        _, _, exact = _sigma_brute(x, y)
+        if exact:
+            assert oracle.sigma(x, y)["S2"] == case["easy"], (x, y)
+            n += 1
+    assert n >= 3
