@@ -89,7 +89,7 @@ class CpxServerStats(ctypes.Structure):
 # every symbol include/cudapix.h declares (checked by the CPU tests)
 EXPORTS = [
     "cpx_opts_default", "cpx_ctx_create", "cpx_ctx_destroy", "cpx_strerror",
-    "cpx_ctx_last_error", "cpx_choose_y", "cpx_ctx_device_bytes",
+    "cpx_ctx_last_error", "cpx_choose_y", "cpx_ctx_device_bytes", "cpx_debug_fail_alloc",
     "cpx_pi_u64", "cpx_pi_u128",
     "cpx_p2_u64", "cpx_p2_u128", "cpx_div_x_by_y",
     "cpx_ordinary_u64", "cpx_ordinary_u128",
@@ -143,6 +143,7 @@ def lib():
         "cpx_ctx_last_error": (ctypes.c_char_p, [vp]),
         "cpx_choose_y": (ctypes.c_int, [vp, CpxU128, P(U64)]),
         "cpx_ctx_device_bytes": (U64, [vp]),
+        "cpx_debug_fail_alloc": (ctypes.c_int, [ctypes.c_int]),
         "cpx_pi_u64": (ctypes.c_int, [vp, U64, U64, ctypes.c_uint32, P(U64), P(CpxTerms)]),
         "cpx_pi_u128": (ctypes.c_int, [vp, CpxU128, U64, ctypes.c_uint32, P(CpxU128), P(CpxTerms)]),
         "cpx_p2_u64": (ctypes.c_int, [vp, U64, U64, P(U64)]),
@@ -769,6 +770,14 @@ def _hip():
 
 
 _default = None
+
+
+def debug_fail_alloc(n):
+    """Test hook: the n-th device allocation of the library from now on fails
+    as on an exhausted device (0 disarms)."""
+    rc = lib().cpx_debug_fail_alloc(int(n))
+    if rc:
+        raise CpxError(rc, "debug_fail_alloc")
 
 
 def default_context():

@@ -22,6 +22,43 @@
 #include <condition_variable>
 #include <functional>
 #include <mutex>
+#ifndef CPX_NO_ROCTX
+#include <rocprofiler-sdk-roctx/roctx.h>
+#endif
+
+// A named range in a rocprofv3 marker trace (--marker-trace) around a host
+// phase of a solve: table builds, planning, plan upload, kernel enqueue.
+struct CpxRange {
+#ifndef CPX_NO_ROCTX
+  explicit CpxRange(const char *name) { roctxRangePushA(name); }
+  ~CpxRange() { roctxRangePop(); }
+#else
+  explicit CpxRange(const char *) {}
+#endif
+};
+
+// Test hook (cpx_debug_fail_alloc): the n-th device allocation from now
+// fails with hipErrorOutOfMemory, as a full device would.
+static std::atomic<int> g_alloc_fail{0};
+
+static hipError_t cpx_dev_malloc(void **p, size_t bytes)
+{
+  int n = g_alloc_fail.load();
+  while (n > 0 && !g_alloc_fail.compare_exchange_weak(n, n - 1)) {
+  }
+  if (n == 1) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  return hipMalloc(p, bytes);
+}
+
+extern "C" int cpx_debug_fail_alloc(int n)
+{
+  if (n < 0) return CPX_EINVAL;
+  g_alloc_fail.store(n);
+  return CPX_OK;
+}
 #include <thread>
 #include <unistd.h>
 
@@ -63,7 +100,7 @@ static int dbuf_reserve(cpx_ctx *c, DBuf<T> &b, size_t n, uint64_t *acct = nullp
     b.cap = 0;
   }
   size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-  hipError_t e = hipMalloc((void **)&b.p, bytes);
+  hipError_t e = cpx_dev_malloc((void **)&b.p, bytes);
   if (e != hipSuccess) { b.p = nullptr; (void)hipGetLastError(); return hip_fail(c, e, "hipMalloc"); }
   b.cap = std::max<size_t>(n, 1);
   if (acct) *acct += b.bytes();
@@ -257,6 +294,7 @@ bool tables_cover(const cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 
 int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 {
+  CpxRange range("cpx:tables");
   int rc;
   if ((rc = build_bootstrap(c))) return rc;
   if ((rc = build_wheels(c, 0))) return rc;
@@ -733,6 +771,7 @@ static unsigned plan_threads(size_t nq)
 static int build_plan(cpx_ctx *c, const std::vector<HostQuery> &hq, uint32_t mask, Plan &pl,
                       bool stream = false, const Caps *base = nullptr)
 {
+  CpxRange range("cpx:plan");
   pl.stream = stream;
   pl.base = base ? *base : Caps{};
   const Caps &B = pl.base;
@@ -905,8 +944,8 @@ static int ensure_arena_caps(cpx_ctx *c, BatchSlot &S, const Caps &need, uint64_
   if (budget && L.plan_bytes + L.out_bytes > budget)
     return fail(ctx_fail(c, CPX_ENOMEM, "batch state exceeds the HBM budget"));
   hipError_t e;
-  if ((e = hipMalloc(&S.d_plan, L.plan_bytes)) != hipSuccess) { S.d_plan = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(plan)")); }
-  if ((e = hipMalloc(&S.d_out, L.out_bytes)) != hipSuccess) { S.d_out = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(out)")); }
+  if ((e = cpx_dev_malloc(&S.d_plan, L.plan_bytes)) != hipSuccess) { S.d_plan = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(plan)")); }
+  if ((e = cpx_dev_malloc(&S.d_out, L.out_bytes)) != hipSuccess) { S.d_out = nullptr; (void)hipGetLastError(); return fail(hip_fail(c, e, "hipMalloc(out)")); }
   if ((e = hipHostMalloc(&S.h_plan, L.plan_bytes, hipHostMallocDefault)) != hipSuccess) { S.h_plan = nullptr; return fail(hip_fail(c, e, "hipHostMalloc")); }
   if ((e = hipHostMalloc(&S.h_res, kk.q * sizeof(ResultDev), hipHostMallocDefault)) != hipSuccess) { S.h_res = nullptr; return fail(hip_fail(c, e, "hipHostMalloc")); }
   S.L = L;
@@ -1060,6 +1099,7 @@ static void pack_stages(const cpx_ctx *c, const Layout &L, const Plan &pl, char 
 // upload stream; the batch stream waits for it (event ev[4]).
 static int upload_plan(cpx_ctx *c, BatchSlot &S, const Plan &pl)
 {
+  CpxRange range("cpx:upload");
   const Layout &L = S.L;
   char *H = (char *)S.h_plan;
   pack_plan(L, pl, H);
@@ -1110,6 +1150,8 @@ static const uint32_t GRID_FIN = 16384;
 //   part 2: hard-leaf carry, combine
 static int enqueue_part(cpx_ctx *c, const BatchDev &b, int part, uint32_t nres_words)
 {
+  CpxRange range(part == 0 ? "cpx:enqueue setup, ordinary, trivial"
+                 : part == 1 ? "cpx:enqueue segment kernel" : "cpx:enqueue carry, combine");
   if (part == 0) {
     HIPCHK(launch_zero_u32((uint32_t *)b.results, nres_words, c->stream));
     HIPCHK(launch_zero_u32(b.tile_head, 1, c->stream));

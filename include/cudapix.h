@@ -383,6 +383,13 @@ int cpx_selftest_wave(cpx_ctx *ctx, uint64_t seed, uint32_t nwaves, uint32_t *ba
  * with HIP events: the denominators of the roofline fractions. */
 int cpx_probe_peaks(cpx_ctx *ctx, double *lds_bytes_per_s, double *int32_ops_per_s);
 
+/* ---- test hook (no counterpart in the reference) ------------------------
+ * The n-th device allocation the library makes from now on (tables, batch
+ * arenas) fails as an exhausted device would (hipErrorOutOfMemory), so that
+ * a test can check the clean CPX_ENOMEM and that nothing leaks; 0 disarms.
+ * Process-wide; for tests only. */
+int cpx_debug_fail_alloc(int n);
+
 #ifdef __cplusplus
 }
 #endif

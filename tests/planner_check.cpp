@@ -12,6 +12,7 @@
 // stdin: one batch per line, "mask p2mode x[:y] x[:y] ..." (decimal x < 2^100).
 // stdout: one line per batch, "ok <queries> <tiles> <plan_bytes> <out_bytes>
 // <estimate_bytes_sum>", or "FAIL <what>" (and exit status 1).
+#define CPX_NO_ROCTX 1
 #include "../cudapix_amd/csrc/engine.hip"
 #include <cstdio>
 #include <iostream>

@@ -205,3 +205,41 @@ def test_device_bytes_keep_the_resident_wheels():
         b2 = fc.device_bytes()
     assert b1 == b2
     assert b1 >= 8 * 4849845
+
+
+def _hip_free_bytes():
+    import ctypes
+    h = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert h.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_allocation_failure_is_clean():
+    """A device allocation that fails (forced from the host on the 1st, 2nd,
+    3rd, 5th and 8th allocation of a solve that builds tables and a batch
+    arena) gives CPX_ENOMEM, the context then solves correctly, and after
+    the context is destroyed the device holds no more memory than before."""
+    import cudapix_amd as C
+    want = 346065536839
+    free0 = _hip_free_bytes()
+    failed = 0
+    with C.Context(device=0, fresh_tables=True) as fc:
+        for n in (1, 2, 3, 5, 8):
+            C.debug_fail_alloc(n)
+            try:
+                v = fc.pi(10 ** 13)
+                assert v == want                   # the solve needed fewer allocations
+            except C.CpxError as e:
+                assert e.code == -2, e             # CPX_ENOMEM
+                failed += 1
+            finally:
+                C.debug_fail_alloc(0)
+            assert fc.pi(10 ** 13) == want
+        b_after = fc.device_bytes()
+    with C.Context(device=0, fresh_tables=True) as ref:
+        assert ref.pi(10 ** 13) == want
+        b_ref = ref.device_bytes()
+    assert failed >= 3
+    assert b_after == b_ref
+    assert _hip_free_bytes() >= free0 - (64 << 20)
