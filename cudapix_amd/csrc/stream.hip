@@ -24,6 +24,7 @@ enum : uint32_t { SW_NONE = 0, SW_COPY, SW_COMB, SW_FIN, SW_MAIN, SW_EXIT };
 struct StreamWork {
   uint32_t kind, slot;
   uint64_t idx;
+  uint32_t n;              // carry or combine units [idx, idx + n)
 };
 
 // Every memory operation of this wave has completed before the next one
@@ -102,35 +103,65 @@ static __device__ __forceinline__ bool slot_of(const StreamPub *pub, uint32_t ns
 // host may reuse the record, so a head read before a record's reuse no longer
 // matches at the compare-and-swap.  A claim at or past the count (read again
 // after the claim) is no work.
-static __device__ __forceinline__ bool claim_phase(uint64_t *head, const uint64_t *count, uint64_t *idx)
+// A claim takes a run of up to (count / (2 * grid), at most 64) units: a
+// cohort's carry has thousands of units of a few microseconds each, and one
+// compare-and-swap (plus its accounting) per unit made the claims, not the
+// units, the cost of the phase.
+static __device__ __forceinline__ bool claim_phase(uint64_t *head, const uint64_t *count, uint64_t *idx,
+                                                   uint32_t *n)
 {
   for (;;) {
     const uint64_t v = ld_dev(head);
     const uint32_t i = (uint32_t)v;
     if (i == 0xffffffffu) return false;
     order_acquire();
-    if (i >= ld_sys(count)) return false;
-    if (cas_dev(head, v, v + 1)) {
+    const uint64_t cnt = ld_sys(count);
+    if (i >= cnt) return false;
+    uint64_t k = cnt / (2ull * gridDim.x);
+    k = k < 1 ? 1 : (k > 64 ? 64 : k);
+    if (k > cnt - i) k = cnt - i;
+    if (cas_dev(head, v, v + k)) {
       order_acquire();
       if (i >= ld_sys(count)) return false;
       *idx = i;
+      *n = (uint32_t)k;
       return true;
     }
   }
 }
 
-// Claim one index below the published end from a FIFO head.
-static __device__ __forceinline__ bool claim_fifo(uint64_t *head, uint64_t end, uint64_t *idx)
+// Claim indices [idx, idx + n) below the published end from a FIFO head,
+// n <= kmax (1 for copy units).
+static __device__ __forceinline__ bool claim_fifo(uint64_t *head, uint64_t end, uint64_t *idx,
+                                                  uint64_t kmax = 1, uint64_t *n = nullptr)
 {
   for (;;) {
     const uint64_t h = ld_dev(head);
     if (h >= end) return false;
-    if (cas_dev(head, h, h + 1)) { *idx = h; return true; }
+    const uint64_t k = min(kmax, end - h);
+    if (cas_dev(head, h, h + k)) {
+      *idx = h;
+      if (n) *n = k;
+      return true;
+    }
   }
 }
 
-// Thread 0: the block's next unit of work, in priority order.
-static __device__ __attribute__((noinline)) void stream_find(const StreamArgs &a, StreamWork &w)
+// Main tiles a block claims at once: every block contends for the one head
+// (each claim a compare-and-swap at device scope), so while many tiles are
+// left a claim takes a run of up to 16 consecutive tiles (1 near the end,
+// where the longest-first order leaves the short tiles to balance the grid).
+static __device__ __forceinline__ uint64_t main_claim_run(uint64_t left)
+{
+  const uint64_t k = left / (4ull * gridDim.x);
+  return k < 1 ? 1 : (k > 16 ? 16 : k);
+}
+
+// Thread 0: the block's next unit of work, in priority order.  A claimed run
+// of main tiles [v, v + n) comes back as its first tile, with *run_next and
+// *run_end set to the rest (the caller takes those before looking again).
+static __device__ __attribute__((noinline)) void stream_find(const StreamArgs &a, StreamWork &w,
+                                                             uint64_t *run_next, uint64_t *run_end)
 {
   StreamCtl *ctl = a.ctl;
   const StreamPub *pub = a.pub;
@@ -155,18 +186,23 @@ static __device__ __attribute__((noinline)) void stream_find(const StreamArgs &a
   }
   // combine, then carry units of cohorts whose earlier phase is done (units
   // are numbered within the cohort; its entries start at the bases)
+  uint32_t k;
   for (j = 0; j < a.nslots; j++)
-    if (claim_phase(&ctl->comb_head[j], &pub->rec[j].n_comb, &v)) {
-      w = {SW_COMB, j, ld_sys(&pub->rec[j].q_base) + v};
+    if (claim_phase(&ctl->comb_head[j], &pub->rec[j].n_comb, &v, &k)) {
+      w = {SW_COMB, j, ld_sys(&pub->rec[j].q_base) + v, k};
       return;
     }
   for (j = 0; j < a.nslots; j++)
-    if (claim_phase(&ctl->fin_head[j], &pub->rec[j].n_fin, &v)) {
-      w = {SW_FIN, j, ld_sys(&pub->rec[j].fin_base) + v};
+    if (claim_phase(&ctl->fin_head[j], &pub->rec[j].n_fin, &v, &k)) {
+      w = {SW_FIN, j, ld_sys(&pub->rec[j].fin_base) + v, k};
       return;
     }
-  if (claim_fifo(&ctl->main_head, me, &v)) {
-    st_sys(&a.mb->main_claimed, v + 1);
+  uint64_t n = 1;
+  const uint64_t h0 = ld_dev(&ctl->main_head);
+  if (claim_fifo(&ctl->main_head, me, &v, main_claim_run(me > h0 ? me - h0 : 0), &n)) {
+    st_sys(&a.mb->main_claimed, v + n);
+    *run_next = v + 1;
+    *run_end = v + n;
     uint64_t loc;
     if (slot_of(pub, a.nslots, 1, v, &j, &loc)) {
       w = {SW_MAIN, j, ld_sys(&pub->rec[j].tile_base) + loc};
@@ -207,13 +243,27 @@ static __device__ __attribute__((noinline)) void stream_copy(const StreamArgs &a
     const uint64_t lo = max(b0, acc), hi = min(b1, acc + len);
     if (lo < hi) {
       // bytes [e0, e1) (multiples of 4): 4-byte moves up to an 8-byte
-      // boundary and after the last one, 8-byte moves between
+      // boundary and after the last one, 8-byte moves between.  The loads
+      // stay system-scope (what makes the host's writes visible to a running
+      // launch); eight of them per lane are issued before their stores, so
+      // a lane has eight PCIe round trips in flight instead of one.
       const uint64_t e0 = off + (lo - acc), e1 = off + (hi - acc);
       const uint64_t a8 = min(e1, (e0 + 7) & ~7ull), z8 = max(a8, e1 & ~7ull);
       for (uint64_t o = e0 + 4ull * t; o < a8; o += 4ull * T)
         st_out((uint32_t *)(dst + o), ld_sys32((const uint32_t *)(src + o)));
-      for (uint64_t o = a8 + 8ull * t; o < z8; o += 8ull * T)
-        st_out((uint64_t *)(dst + o), ld_sys((const uint64_t *)(src + o)));
+      const uint64_t *s8 = (const uint64_t *)(src + a8);
+      uint64_t *d8 = (uint64_t *)(dst + a8);
+      const uint64_t nw = (z8 - a8) >> 3;
+      constexpr uint32_t U = 8;
+      uint64_t i = t;
+      for (; i + (U - 1) * (uint64_t)T < nw; i += U * (uint64_t)T) {
+        uint64_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) v[u] = ld_sys(s8 + i + u * (uint64_t)T);
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) st_out(d8 + i + u * (uint64_t)T, v[u]);
+      }
+      for (; i < nw; i += T) st_out(d8 + i, ld_sys(s8 + i));
       for (uint64_t o = z8 + 4ull * t; o < e1; o += 4ull * T)
         st_out((uint32_t *)(dst + o), ld_sys32((const uint32_t *)(src + o)));
     }
@@ -244,9 +294,13 @@ static __device__ __attribute__((noinline)) void stream_term(const BatchDev &bd,
 }
 
 // One main tile: the batch kernel's segment, easy-leaf and P2-lookup tiles,
-// and the ordinary and trivial descriptors.
+// and the ordinary and trivial descriptors.  Out of line: inlined into the
+// kernel loop, whose values live across the calls to the other units, the
+// tile code spilled registers to scratch inside its inner loops (hundreds of
+// scratch accesses; the tiles ran several times slower than in the batch
+// kernel); as a function of its own it is allocated like the batch kernel.
 template <int T, int K>
-static __device__ __forceinline__ void stream_main(const BatchDev &bd, const BatchDev *bd_mem, uint64_t ti,
+static __device__ __attribute__((noinline)) void stream_main(const BatchDev &bd, const BatchDev *bd_mem, uint64_t ti,
                                                    uint32_t *lds, SegShared<T, K> &sh)
 {
   const TileDev tile = bd.tiles[ti];
@@ -286,13 +340,13 @@ static __device__ __attribute__((noinline)) void stream_account(const StreamArgs
       else st_dev(&ctl->comb_head[j], seq << 32);
     }
   } else if (w.kind == SW_FIN) {
-    const uint32_t n = add_dev32(&ctl->fin_done[j], 1) + 1;
+    const uint32_t n = add_dev32(&ctl->fin_done[j], w.n) + w.n;
     if (n == ld_sys(&r.n_fin)) {
       order_release();
       st_dev(&ctl->comb_head[j], seq << 32);
     }
   } else if (w.kind == SW_COMB) {
-    const uint32_t n = add_dev32(&ctl->comb_done[j], 1) + 1;
+    const uint32_t n = add_dev32(&ctl->comb_done[j], w.n) + w.n;
     if (n == ld_sys(&r.n_comb)) {
       // the cohort is complete: report, reset the slot, then signal
       order_acquire();
@@ -321,9 +375,23 @@ __global__ __launch_bounds__(T, 4) void k_stream(const StreamArgs a)
   __shared__ StreamWork w;
   uint64_t idle_since = wall_clock64();
   uint32_t backoff = 1;
+  // thread 0: the rest of the block's claimed run of main tiles; all of them
+  // lie below the main end read when the run was claimed, so their cohorts'
+  // copy units had all been claimed by then
+  uint64_t run_next = 0, run_end = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      stream_find(a, w);
+      uint32_t js;
+      uint64_t loc;
+      if (run_next < run_end && slot_of(a.pub, a.nslots, 1, run_next, &js, &loc)) {
+        w = {SW_MAIN, js, ld_sys(&a.pub->rec[js].tile_base) + loc};
+        run_next++;
+      } else if (run_next < run_end) {
+        set_error(a, 4u);
+        w.kind = SW_EXIT;
+      } else {
+        stream_find(a, w, &run_next, &run_end);
+      }
       if (w.kind == SW_NONE && wall_clock64() - idle_since > a.idle_ticks) {
         // the host went quiet without closing: leave, flagging any cohort left behind
         if (ld_dev(&a.ctl->completed) < ld_sys(&a.pub->cohorts)) set_error(a, 2u);
@@ -369,14 +437,20 @@ __global__ __launch_bounds__(T, 4) void k_stream(const StreamArgs a)
       if (w.kind == SW_EXIT) break;
       stream_main<T, K>(a.bd, a.bd_mem, wk.idx, lds, sh);
     } else if (wk.kind == SW_FIN) {
-      stream_fin<T>(*a.bd_mem, wk.idx, lds);
+      for (uint32_t u = 0; u < wk.n; u++) {
+        if (u) __syncthreads();
+        stream_fin<T>(*a.bd_mem, wk.idx + u, lds);
+      }
     } else {
-      stream_comb<T>(*a.bd_mem, wk.idx, lds);
+      for (uint32_t u = 0; u < wk.n; u++) {
+        if (u) __syncthreads();
+        stream_comb<T>(*a.bd_mem, wk.idx + u, lds);
+      }
     }
     // every wave's stores of the unit have completed before it is counted
     order_release();
     __syncthreads();
-    if (threadIdx.x == 0) stream_account(a, {wk.kind, j, wk.idx}, wall_clock64());
+    if (threadIdx.x == 0) stream_account(a, {wk.kind, j, wk.idx, wk.n}, wall_clock64());
     idle_since = wall_clock64();
     __syncthreads();
   }
