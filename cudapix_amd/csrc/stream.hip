@@ -317,10 +317,38 @@ static __device__ __attribute__((noinline)) void stream_main(const BatchDev &bd,
   }
 }
 
+// Thread 0's copy of the record of the cohort whose main tiles it runs.
+struct StreamCohort {
+  uint64_t start = 0, count = 0;   // its main tiles [start, start + count)
+  uint64_t tile_base = 0, seq = 0, n_main = 0;
+  uint32_t slot = 0, ncopy = 0;
+  __device__ bool holds(uint64_t v) const { return count && v >= start && v < start + count; }
+  // the record of the slot whose main range holds v (system-scope loads)
+  __device__ bool load(const StreamArgs &a, uint64_t v)
+  {
+    uint64_t loc;
+    if (!slot_of(a.pub, a.nslots, 1, v, &slot, &loc)) {
+      count = 0;
+      return false;
+    }
+    const StreamRec &r = a.pub->rec[slot];
+    const uint64_t mr = ld_sys(&r.main_range), cr = ld_sys(&r.copy_range);
+    const uint64_t mask = (1ull << CPX_STREAM_NBITS) - 1;
+    start = mr >> CPX_STREAM_NBITS;
+    count = mr & mask;
+    tile_base = ld_sys(&r.tile_base);
+    seq = ld_sys(&r.seq);
+    n_main = ld_sys(&r.n_main);
+    ncopy = (uint32_t)(cr & mask);
+    return holds(v);
+  }
+};
+
 // Thread 0, after every wave's stores of the unit completed: counts the unit
 // done and opens its cohort's next phase when it was the last.
 static __device__ __attribute__((noinline)) void stream_account(const StreamArgs &a, const StreamWork &w,
-                                                                uint64_t now)
+                                                                uint64_t now, uint64_t seq_main,
+                                                                uint64_t n_main)
 {
   StreamCtl *ctl = a.ctl;
   StreamMailbox *mb = a.mb;
@@ -330,16 +358,20 @@ static __device__ __attribute__((noinline)) void stream_account(const StreamArgs
     add_dev32(&ctl->copied[j], 1);
     return;
   }
-  const uint64_t seq = ld_sys(&r.seq);
   if (w.kind == SW_MAIN) {
+    // (seq and n_main: the block's cached copy of the cohort's record)
+    const uint64_t seq = seq_main;
     __hip_atomic_fetch_max(&ctl->t_last[j], now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t n = add_dev32(&ctl->main_done[j], 1) + 1;
-    if (n == ld_sys(&r.n_main)) {
+    if (n == n_main) {
       order_release();
       if (ld_sys(&r.n_fin)) st_dev(&ctl->fin_head[j], seq << 32);
       else st_dev(&ctl->comb_head[j], seq << 32);
     }
-  } else if (w.kind == SW_FIN) {
+    return;
+  }
+  const uint64_t seq = ld_sys(&r.seq);
+  if (w.kind == SW_FIN) {
     const uint32_t n = add_dev32(&ctl->fin_done[j], w.n) + w.n;
     if (n == ld_sys(&r.n_fin)) {
       order_release();
@@ -379,18 +411,27 @@ __global__ __launch_bounds__(T, 4) void k_stream(const StreamArgs a)
   // lie below the main end read when the run was claimed, so their cohorts'
   // copy units had all been claimed by then
   uint64_t run_next = 0, run_end = 0;
+  // thread 0: the record of the cohort whose main tiles it takes (read once
+  // per cohort and run, not per tile: a record does not change while the
+  // cohort has tiles this block claimed, and main indices only grow, so a
+  // range cached from a completed cohort matches no later index)
+  StreamCohort co{};
   for (;;) {
     if (threadIdx.x == 0) {
-      uint32_t js;
-      uint64_t loc;
-      if (run_next < run_end && slot_of(a.pub, a.nslots, 1, run_next, &js, &loc)) {
-        w = {SW_MAIN, js, ld_sys(&a.pub->rec[js].tile_base) + loc};
-        run_next++;
-      } else if (run_next < run_end) {
-        set_error(a, 4u);
-        w.kind = SW_EXIT;
+      if (run_next < run_end) {
+        if (!co.holds(run_next) && !co.load(a, run_next)) {
+          set_error(a, 4u);
+          w.kind = SW_EXIT;
+        } else {
+          w = {SW_MAIN, co.slot, co.tile_base + (run_next - co.start)};
+          run_next++;
+        }
       } else {
         stream_find(a, w, &run_next, &run_end);
+        if (w.kind == SW_MAIN && !co.holds(run_next - 1) && !co.load(a, run_next - 1)) {
+          set_error(a, 4u);
+          w.kind = SW_EXIT;
+        }
       }
       if (w.kind == SW_NONE && wall_clock64() - idle_since > a.idle_ticks) {
         // the host went quiet without closing: leave, flagging any cohort left behind
@@ -416,8 +457,7 @@ __global__ __launch_bounds__(T, 4) void k_stream(const StreamArgs a)
     } else if (wk.kind == SW_MAIN) {
       if (threadIdx.x == 0) {
         // the cohort's copy units were claimed before this tile; wait for them
-        const uint32_t ncopy =
-            (uint32_t)(ld_sys(&a.pub->rec[j].copy_range) & ((1ull << CPX_STREAM_NBITS) - 1));
+        const uint32_t ncopy = co.ncopy;
         const uint64_t t0 = wall_clock64();
         w.kind = SW_MAIN;
         while (__hip_atomic_load(&a.ctl->copied[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ncopy) {
@@ -450,7 +490,7 @@ __global__ __launch_bounds__(T, 4) void k_stream(const StreamArgs a)
     // every wave's stores of the unit have completed before it is counted
     order_release();
     __syncthreads();
-    if (threadIdx.x == 0) stream_account(a, {wk.kind, j, wk.idx, wk.n}, wall_clock64());
+    if (threadIdx.x == 0) stream_account(a, {wk.kind, j, wk.idx, wk.n}, wall_clock64(), co.seq, co.n_main);
     idle_since = wall_clock64();
     __syncthreads();
   }
