@@ -294,13 +294,12 @@ static __device__ __attribute__((noinline)) void stream_term(const BatchDev &bd,
 }
 
 // One main tile: the batch kernel's segment, easy-leaf and P2-lookup tiles,
-// and the ordinary and trivial descriptors.  Out of line: inlined into the
-// kernel loop, whose values live across the calls to the other units, the
-// tile code spilled registers to scratch inside its inner loops (hundreds of
-// scratch accesses; the tiles ran several times slower than in the batch
-// kernel); as a function of its own it is allocated like the batch kernel.
+// and the ordinary and trivial descriptors.  Inlined: out of line, the
+// tile code would see the LDS arrays through generic pointers (flat
+// accesses that wait on global loads too) and read the arena record
+// through memory instead of kernel arguments.
 template <int T, int K>
-static __device__ __attribute__((noinline)) void stream_main(const BatchDev &bd, const BatchDev *bd_mem, uint64_t ti,
+static __device__ __forceinline__ void stream_main(const BatchDev &bd, const BatchDev *bd_mem, uint64_t ti,
                                                    uint32_t *lds, SegShared<T, K> &sh)
 {
   const TileDev tile = bd.tiles[ti];
