@@ -231,8 +231,8 @@ def cpu_baseline(xs, full=False, budget_s=90.0):
     """The oracle (C + OpenMP, rebuilt here with -O3 -march=native) on the
     host: 1 warm-up and the median of 3 for every timing.
       * queries/s on a stratified sample of the 512-query list (the median x
-        of each of 8 strata of 64 sorted queries; a stratum's time stands for
-        its 64 queries);
+        of each of 16 strata of 32 sorted queries; a stratum's time stands for
+        its 32 queries);
       * C1 (x = 1e12, alpha 2, c = 6): P2, ordinary, trivial, easy and hard
         terms, and the brute segmented-sieve count of primes below 1e12 timed
         on the window [1e12 - 2^33, 1e12] and scaled by 1e12 / 2^33 (with
@@ -243,7 +243,7 @@ def cpu_baseline(xs, full=False, budget_s=90.0):
     hi = host_info()
     th = hi["threads"]
     srt = sorted(xs)
-    k = 8
+    k = 16
     per = len(srt) // k
     sample = [srt[i * per + per // 2] for i in range(k)]
     O.pi_dr(sample[0], O.choose_y(sample[0], cpu_alpha(sample[0])), 6, th)      # warm-up
@@ -256,7 +256,7 @@ def cpu_baseline(xs, full=False, budget_s=90.0):
         est += t * per
         strata.append({"x": x, "y": y, "s": round(t, 4), "reps": reps})
     out = {"value": round(len(srt) / est, 4), "unit": "queries/s", "cores": th, "kind": "port",
-           "sample": "median x of each of 8 strata of the sorted 512-query list, each timed as the "
+           "sample": "median x of each of 16 strata of the sorted 512-query list, each timed as the "
                      "median of 3 (1 warm-up before the first), alpha = clamp(log10 x - 5, 1, 12); "
                      "%.1f s of CPU" % spent,
            "build": "gcc " + flags, "host": hi, "strata": strata}
