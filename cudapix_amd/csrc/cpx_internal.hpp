@@ -68,11 +68,13 @@ struct QueryDev {
 // One stage of the hard-leaf sieve: p_b, floor(x / p_b) and constants that
 // turn the per-stage divisions into multiplications (computed once per query).
 struct StageDev {
-  uint64_t xp;         // floor(x / p)
+  uint64_t xp;         // floor(x / p), low 64 bits
   double inv_p;        // 1 / p rounded to f64
   uint64_t magic;      // ceil(2^64 / p): a mod p for 32-bit a (Lemire, Kaser, Kurz 2019)
   uint32_t p;
   uint32_t ylo;        // floor(y / p)
+  uint32_t xph;        // floor(x / p) >> 64 (nonzero only for the first stages of x above 2^68)
+  uint32_t pad;
 };
 
 // floor(a / d) from a precomputed inv ~ 1/d: exact when a < 2^53 or the
@@ -84,6 +86,26 @@ __host__ __device__ __forceinline__ uint64_t cpx_div_recip(uint64_t a, uint64_t 
   if (r < 0) q--;
   else if ((uint64_t)r >= d) q++;
   return q;
+}
+
+// floor(a / d) for a = ah * 2^64 + al (ah < 2^32) whose quotient is below
+// 2^50 (checked by the caller with cpx_quot_below_2_50), 2 <= d < 2^62: the
+// f64 estimate of a * inv is within one of the quotient, and the remainder
+// al - q * d taken modulo 2^64 is the true remainder (it lies in (-d, 2d)).
+__host__ __device__ __forceinline__ uint64_t cpx_div_recip_w(uint64_t al, uint32_t ah, uint64_t d, double inv)
+{
+  const double a = (double)al + (double)ah * 0x1p64;
+  uint64_t q = (uint64_t)(a * inv);
+  int64_t r = (int64_t)(al - q * d);
+  if (r < 0) q--;
+  else if ((uint64_t)r >= d) q++;
+  return q;
+}
+
+// (ah * 2^64 + al) / d < 2^50, for d > 0.
+__host__ __device__ __forceinline__ bool cpx_quot_below_2_50(uint64_t al, uint32_t ah, uint64_t d)
+{
+  return ((((uint64_t)ah) << 14) | (al >> 50)) < d;
 }
 
 // a mod p for 32-bit a with magic = ceil(2^64 / p).

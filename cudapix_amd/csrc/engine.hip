@@ -210,8 +210,10 @@ int make_host_query(cpx_ctx *c, u128 x, uint64_t y, uint32_t cc, HostQuery *hq)
   uint32_t k = 0;
   while (k < 8 && sp[k] <= h.qrtx) k++;
   h.c = cc < k ? cc : k;
-  // floor(x / p_{c+1}) must fit 64 bits for the stage table
-  if (h.c < 8 && (x / sp[h.c]) >> 64) return ctx_fail(c, CPX_ERANGE, "x / p_(c+1) must fit 64 bits; raise c");
+  // the easy leaves divide x / p for p > x^(1/4) as 64-bit values (x < ~2^85);
+  // the hard segment tiles hold x / p in 80 bits
+  if ((x / h.qrtx) >> 64) return ctx_fail(c, CPX_ERANGE, "x / x^(1/4) must fit 64 bits");
+  if ((x / sp[h.c < 8 ? h.c : 7]) >> 80) return ctx_fail(c, CPX_ERANGE, "x / p_(c+1) must be below 2^80");
   *hq = h;
   return CPX_OK;
 }
@@ -519,10 +521,10 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   // hard rows: active stages per segment and, for the low (leaf-dense)
   // segments, an estimate of the leaves to evaluate (replicas are chosen per
   // group, from the members' leaves of the segment)
-  std::vector<uint64_t> xps;
+  std::vector<double> xps;
   if (d.hard_nseg) {
     xps.resize(d.nstage);
-    for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (uint64_t)(h.x / c->h_primes[h.c + k]);
+    for (uint32_t k = 0; k < d.nstage; k++) xps[k] = (double)(h.x / c->h_primes[h.c + k]);
   }
   const double leaf_target = 256.0 * T;
   uint32_t sparse_run = 0;
@@ -1089,7 +1091,10 @@ static void pack_stages(const cpx_ctx *c, const Layout &L, const Plan &pl, char 
       d.ylo = (uint32_t)(q.y / p);
       d.inv_p = 1.0 / (double)p;
       d.magic = ~0ull / p + 1;
-      d.xp = (uint64_t)(x / p);
+      const u128 xq = x / p;
+      d.xp = (uint64_t)xq;
+      d.xph = (uint32_t)(xq >> 64);
+      d.pad = 0;
       st[q.stage_off + k] = d;
     }
   });

@@ -82,7 +82,8 @@ struct SegShared {
     struct {                     // hard: chunk table and per-wave counters
       // per item i = k * gs + g (stage k of the chunk, member g; gs member
       // slots, a power of two)
-      uint64_t xp[HARD_SCH];     // floor(x_g / p_k)
+      uint64_t xp[HARD_SCH];     // floor(x_g / p_k), low 64 bits
+      uint16_t xph[HARD_SCH];    // ... and bits 64-79 (nonzero only for x above 2^68)
       uint32_t cbase[HARD_SCH];  // leaf candidates: mtop/2 (dense) or first prime index
       uint32_t ccnt[HARD_SCH];   // candidates | HARD_WIDE | HARD_SPARSE
       int32_t wm[HARD_SCH][T / CPX_WAVE];    // sum of mu of each wave's leaves
@@ -223,19 +224,22 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
     uint32_t ncand = 0, cbase = 0;
     bool sparse = false;
     uint64_t xp = 0;
+    uint32_t xph = 0;
     if (s0 + k < m.nst) {
       const StageDev sd = bd.stages[m.stage_off + s0 + k];
       const uint64_t p = sd.p, y = m.y;
       xp = sd.xp;
+      xph = sd.xph;
       // m range of this segment's leaves of p: x/(p hi) < m <= x/(p lo), m > y/p
       sparse = p * p > y;
       uint64_t mhi = y;
-      if (lo != 0 && (xp >> 50) < lo) {
-        const uint64_t v = cpx_div_recip(xp, lo, inv_lo);
+      if (lo != 0 && cpx_quot_below_2_50(xp, xph, lo)) {
+        const uint64_t v = xph ? cpx_div_recip_w(xp, xph, lo, inv_lo) : cpx_div_recip(xp, lo, inv_lo);
         if (v < mhi) mhi = v;
       }
       uint64_t mlo = ~0ull;
-      if ((xp >> 50) < hi) mlo = cpx_div_recip(xp, hi, inv_hi);
+      if (cpx_quot_below_2_50(xp, xph, hi))
+        mlo = xph ? cpx_div_recip_w(xp, xph, hi, inv_hi) : cpx_div_recip(xp, hi, inv_hi);
       if (mlo < mhi) {
         if (mlo < sd.ylo) mlo = sd.ylo;
         if (sparse && mlo < p) mlo = p;
@@ -253,6 +257,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
       }
     }
     sh.u.st.xp[it] = xp;
+    sh.u.st.xph[it] = (uint16_t)xph;
     sh.u.st.cbase[it] = cbase;
     sh.u.st.ccnt[it] = ncand | (sparse ? HARD_SPARSE : 0u) | (ncand > 0xffffu ? HARD_WIDE : 0u);
     if (ncand) atomicOr(&sh.u.st.lmask[it >> 5], 1u << (it & 31));
@@ -266,8 +271,12 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
     if (w == 0 || !(cc & HARD_NC_MASK) || (cc & HARD_WIDE)) continue;
     const uint32_t ncand = cc & HARD_NC_MASK, cb = sh.u.st.cbase[it];
     const uint64_t xp = sh.u.st.xp[it];
+    const uint32_t xph = sh.u.st.xph[it];
     const uint64_t lo_w = lo + 2 * (uint64_t)w * WB;          // > 0
-    const uint64_t M = cpx_div_recip(xp, lo_w, 1.0 / (double)lo_w);
+    // (a quotient of 2^50 or more exceeds every m <= y: no n below lo_w)
+    const uint64_t M = !cpx_quot_below_2_50(xp, xph, lo_w) ? ~0ull
+                       : xph ? cpx_div_recip_w(xp, xph, lo_w, 1.0 / (double)lo_w)
+                             : cpx_div_recip(xp, lo_w, 1.0 / (double)lo_w);
     uint64_t b;
     if (!(cc & HARD_SPARSE)) {
       const uint64_t mtop = 2 * (uint64_t)cb + 1;            // m = mtop - 2i
@@ -338,8 +347,12 @@ __device__ __forceinline__ void hard_wave_range(const BatchDev &bd, uint64_t y,
   }
   const uint32_t cb = sh.u.st.cbase[k];
   const uint64_t xp = sh.u.st.xp[k];
-  const uint64_t Mlo = lo_w ? cpx_div_recip(xp, lo_w, inv_lo_w) : ~0ull;
-  const uint64_t Mhi = cpx_div_recip(xp, hi_w, inv_hi_w);
+  const uint32_t xph = sh.u.st.xph[k];
+  // (quotients of 2^50 or more exceed every m <= y)
+  const uint64_t Mlo = !(lo_w && cpx_quot_below_2_50(xp, xph, lo_w)) ? ~0ull
+                       : xph ? cpx_div_recip_w(xp, xph, lo_w, inv_lo_w) : cpx_div_recip(xp, lo_w, inv_lo_w);
+  const uint64_t Mhi = !cpx_quot_below_2_50(xp, xph, hi_w) ? ~0ull
+                       : xph ? cpx_div_recip_w(xp, xph, hi_w, inv_hi_w) : cpx_div_recip(xp, hi_w, inv_hi_w);
   if (!(cc & HARD_SPARSE)) {                                 // m = mtop - 2i
     const uint64_t mtop = 2 * (uint64_t)cb + 1;
     const uint64_t a = mtop > Mlo ? (mtop - Mlo + 1) / 2 : 0;
@@ -747,7 +760,8 @@ __device__ __forceinline__ void hard_chunks(const BatchDev &bd, SegShared<T, K> 
             }
             if (sg) {
               const uint64_t xp = sh.u.st.xp[it];
-              const uint64_t n = cpx_div_leaf(xp, (double)xp, m);   // lo_w <= n < hi_w
+              const double xpf = (double)xp + (double)sh.u.st.xph[it] * 0x1p64;
+              const uint64_t n = cpx_div_leaf(xp, xpf, m);   // lo_w <= n < hi_w
               const int32_t cn = (int32_t)count_in_wave<T, K>(lds, sh, wv,
                                                               ((uint32_t)n - (uint32_t)lo_w + 1u) >> 1);
               val = sg > 0 ? cn : -cn;

@@ -42,7 +42,11 @@ __global__ __launch_bounds__(256) void k_setup(BatchDev bd)
       d.ylo = (uint32_t)(q.y / p);
       d.inv_p = 1.0 / (double)p;
       d.magic = ~0ull / p + 1;
-      d.xp = cpx_div128_to64(x, p);
+      uint64_t rem;
+      const u128 xq = cpx_div128_64(x, p, &rem);
+      d.xp = (uint64_t)xq;
+      d.xph = (uint32_t)(xq >> 64);
+      d.pad = 0;
       bd.stages[q.stage_off + s] = d;
     }
   }

@@ -66,3 +66,12 @@ def test_batch_with_u128_member(ctx):
     xs = [2 ** 64] + [10 ** n for n in range(9, 17)]
     want = [KNOWN["pi2"]["64"]] + [KNOWN["pi10"][str(n)] for n in range(9, 17)]
     assert ctx.pi_batch(xs) == want
+
+
+def test_pi_1e21_wide_stage_quotients(ctx):
+    """x = 1e21: x / p exceeds 2^64 for the first hard stages (p < 55), so the
+    segment tiles carry its high bits; the published pi(10^21)."""
+    print("case pi(1e21)", flush=True)
+    t = ctx.pi(10 ** 21, terms=True)
+    print("pi(1e21) y=%d in %.0f ms" % (t["y"], t["ms"]), flush=True)
+    assert t["pi"] == KNOWN["pi10"]["21"]
