@@ -369,31 +369,32 @@ __device__ __forceinline__ void hard_wave_range(const BatchDev &bd, uint64_t y,
 
 // Survivors among the first `gw` bits (0 <= gw <= 64 * 32K) of this wave's
 // part of the segment: in-wave prefix of the owning lane + its words below,
-// from the packed prefix/half-count words the wave published.
+// from the packed prefix/half-count words the wave published.  All K/2 words
+// of the owner's half, the packed word and the partial word are loaded before
+// any is used, so the count costs one LDS round trip; the words at or above
+// the bit are masked out.  (For gw at the end of a lane's bits the partial
+// word is the owner's padding word K, masked to nothing.)
 template <int T, int K>
 __device__ __forceinline__ uint32_t count_in_wave(const uint32_t *lds, const SegShared<T, K> &sh,
                                                   uint32_t wv, uint32_t gw)
 {
-  constexpr uint32_t BPT = 32 * K;
+  constexpr uint32_t BPT = 32 * K, H = K / 2;
   uint32_t ol = gw / BPT;
   uint32_t jb = gw - ol * BPT;
   if (ol >= (uint32_t)CPX_WAVE) { ol = CPX_WAVE - 1; jb = BPT; }
   const uint32_t owner = wv * CPX_WAVE + ol;
+  const bool upper = jb >= BPT / 2;
+  const uint32_t *ow = lds + owner * (K + 1) + (upper ? H : 0u);
+  const uint32_t nf = (jb >> 5) - (upper ? H : 0u);  // whole words below the bit: 0 .. H
+  uint32_t wd[H];
+#pragma unroll
+  for (uint32_t r = 0; r < H; r++) wd[r] = ow[r];
+  const uint32_t part = ow[nf];
   const uint32_t e = sh.ipre[owner];
-  uint32_t c = e & CountPack<K>::PRE_MASK;
-  uint32_t w0 = 0;
-  if (jb >= BPT / 2) { c += e >> CountPack<K>::PRE_BITS; w0 = K / 2; }
-  const uint32_t wix = jb >> 5;
-  const uint32_t *ow = lds + owner * (K + 1);
-  for (uint32_t g = w0; g < wix; g += 4) {
-    const uint32_t x0 = ow[g], x1 = ow[g + 1], x2 = ow[g + 2], x3 = ow[g + 3];
-    c += cpx_popc32(x0);
-    c += g + 1 < wix ? cpx_popc32(x1) : 0u;
-    c += g + 2 < wix ? cpx_popc32(x2) : 0u;
-    c += g + 3 < wix ? cpx_popc32(x3) : 0u;
-  }
-  if (jb & 31u) c += cpx_popc32(ow[wix] & cpx_lowmask32(jb & 31u));
-  return c;
+  uint32_t c = (e & CountPack<K>::PRE_MASK) + (upper ? e >> CountPack<K>::PRE_BITS : 0u);
+#pragma unroll
+  for (uint32_t r = 0; r < H; r++) c += cpx_popc32(r < nf ? wd[r] : 0u);
+  return c + cpx_popc32(part & cpx_lowmask32(jb & 31u));
 }
 
 // This wave's leaves of chunk stage k as one list over the stage's leafy
