@@ -133,34 +133,33 @@ __device__ __forceinline__ uint32_t own_count_lo(const uint32_t *lds, uint32_t t
 // Number of survivors among the first `bit` bits of the segment
 // (0 <= bit <= 32*K*T), given the wave prefixes wp[] held in registers:
 // wave prefix + in-wave prefix of the owning thread + its words below the
-// bit, counted from the nearer half boundary (at most K/2 words, read four
-// at a time so that each group costs one LDS round trip).
+// bit, counted from the nearer half boundary.  The K/2 words of that half,
+// the packed word and the partial word are loaded before any is used (one
+// LDS round trip); words at or above the bit are masked out, and at the end
+// of a thread's bits the partial word is its padding word K, masked to nothing.
 template <int T, int K>
 __device__ __forceinline__ uint32_t count_below(const uint32_t *lds, const SegShared<T, K> &sh,
                                                 const uint32_t (&wp)[T / CPX_WAVE], uint32_t bit)
 {
-  constexpr uint32_t BPT = 32 * K;
+  constexpr uint32_t BPT = 32 * K, H = K / 2;
   uint32_t owner = bit / BPT;
   uint32_t jb = bit - owner * BPT;
   if (owner >= (uint32_t)T) { owner = T - 1; jb = BPT; }
   uint32_t base = 0;
 #pragma unroll
   for (uint32_t w = 0; w < (uint32_t)(T / CPX_WAVE); w++) base = (owner / CPX_WAVE == w) ? wp[w] : base;
+  const bool upper = jb >= BPT / 2;
+  const uint32_t *ow = lds + owner * (K + 1) + (upper ? H : 0u);
+  const uint32_t nf = (jb >> 5) - (upper ? H : 0u);  // whole words below the bit: 0 .. H
+  uint32_t wd[H];
+#pragma unroll
+  for (uint32_t r = 0; r < H; r++) wd[r] = ow[r];
+  const uint32_t part = ow[nf];
   const uint32_t e = sh.ipre[owner];
-  uint32_t c = base + (e & CountPack<K>::PRE_MASK);
-  uint32_t w0 = 0;                                   // first word to scan
-  if (jb >= BPT / 2) { c += e >> CountPack<K>::PRE_BITS; w0 = K / 2; }
-  const uint32_t wix = jb >> 5;                      // words [w0, wix) are whole
-  const uint32_t *ow = lds + owner * (K + 1);
-  for (uint32_t g = w0; g < wix; g += 4) {
-    const uint32_t x0 = ow[g], x1 = ow[g + 1], x2 = ow[g + 2], x3 = ow[g + 3];
-    c += cpx_popc32(x0);
-    c += g + 1 < wix ? cpx_popc32(x1) : 0u;
-    c += g + 2 < wix ? cpx_popc32(x2) : 0u;
-    c += g + 3 < wix ? cpx_popc32(x3) : 0u;
-  }
-  if (jb & 31u) c += cpx_popc32(ow[wix] & cpx_lowmask32(jb & 31u));
-  return c;
+  uint32_t c = base + (e & CountPack<K>::PRE_MASK) + (upper ? e >> CountPack<K>::PRE_BITS : 0u);
+#pragma unroll
+  for (uint32_t r = 0; r < H; r++) c += cpx_popc32(r < nf ? wd[r] : 0u);
+  return c + cpx_popc32(part & cpx_lowmask32(jb & 31u));
 }
 
 template <int T, int K>
