@@ -34,7 +34,7 @@ class CpxOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("c", ctypes.c_uint32), ("alpha", ctypes.c_double),
                 ("seg_threads", ctypes.c_uint32), ("seg_kwords", ctypes.c_uint32),
                 ("hbm_budget", U64), ("fresh_tables", ctypes.c_uint32),
-                ("use_graphs", ctypes.c_uint32)]
+                ("use_graphs", ctypes.c_uint32), ("pass_bytes", U64)]
 
 
 class CpxSigmaTerms(ctypes.Structure):
@@ -58,7 +58,8 @@ class CpxBatchStats(ctypes.Structure):
                 ("ms_tables", ctypes.c_double), ("ms_device", ctypes.c_double),
                 ("ms_segments", ctypes.c_double), ("tiles", U64), ("hard_segments", U64),
                 ("p2_segments", U64), ("seg_threads", ctypes.c_uint32),
-                ("seg_kwords", ctypes.c_uint32)]
+                ("seg_kwords", ctypes.c_uint32), ("passes", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class CpxServerOpts(ctypes.Structure):
@@ -250,13 +251,14 @@ class Context:
     """A device context: resident tables + stream (one per thread of use)."""
 
     def __init__(self, device=0, c=6, alpha=0.0, seg_threads=0, seg_kwords=0, hbm_budget=0,
-                 fresh_tables=False, use_graphs=True):
+                 fresh_tables=False, use_graphs=True, pass_bytes=0):
         L = lib()
         o = CpxOpts()
         L.cpx_opts_default(ctypes.byref(o))
         o.device, o.c, o.alpha = device, c, alpha
         o.seg_threads, o.seg_kwords, o.hbm_budget = seg_threads, seg_kwords, hbm_budget
         o.fresh_tables, o.use_graphs = int(fresh_tables), int(use_graphs)
+        o.pass_bytes = pass_bytes
         h = ctypes.c_void_p()
         rc = L.cpx_ctx_create(ctypes.byref(o), ctypes.byref(h))
         if rc:

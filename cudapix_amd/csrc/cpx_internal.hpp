@@ -204,6 +204,16 @@ struct BatchDev {
   uint32_t *p2_cnt;      uint32_t *p2_np;     uint64_t *p2_loc;   // [row]
   i128 *parts;           // partial sums
   ResultDev *results;
+  // one query's hard segments in passes over segment ranges (its rows of
+  // T and sum(mu) held for one range at a time): this launch's rows are the
+  // segments [pass_r0, pass_r1) of the batch's only query, stored from
+  // element pass_toff of its T and pass_moff of its M; hard_base holds each
+  // stage's survivors in the earlier ranges (the carry blocks read and
+  // advance it), and pass_add makes the carry blocks add to their partial
+  // sums.  Outside passes: r0 = 0, r1 = ~0, offsets 0, hard_base null.
+  uint64_t *hard_base;
+  uint64_t pass_r0, pass_r1, pass_toff, pass_moff;
+  uint32_t pass_add, pass_pad;
 };
 
 // Segment-kernel shape: T threads, K words (32 bits) per thread, so a segment

@@ -73,6 +73,10 @@ struct FinShared {
 };
 
 // Writes the descriptor's carry to bd.parts; every thread of the block calls it.
+// In a run in passes it scans the pass's rows only, starts each stage's
+// prefix from hard_base (the survivors of the earlier passes' segments),
+// writes the advanced totals back, and adds to the partial sum after the
+// first pass: the stages' carries over the passes then sum to the one-launch carry.
 template <uint32_t NT>
 __device__ __forceinline__ void hard_fin_desc(const BatchDev &bd, const BlockDesc &d, FinShared<NT> &fs)
 {
@@ -82,21 +86,23 @@ __device__ __forceinline__ void hard_fin_desc(const BatchDev &bd, const BlockDes
   const uint64_t s0 = d.begin;
   const uint32_t ns = (uint32_t)(d.end - d.begin);           // 1..FS, block-uniform
   i128 carry = 0;
+  // survivors of each stage in the rows before this launch's (runs in passes)
   unsigned long long base[FS];
 #pragma unroll
-  for (uint32_t j = 0; j < FS; j++) base[j] = 0;
+  for (uint32_t j = 0; j < FS; j++) base[j] = bd.hard_base && j < ns ? bd.hard_base[s0 + j] : 0ull;
+  const uint64_t r_end = min(q.hard_nseg, bd.pass_r1);
   // the next chunk's row records are fetched while this chunk is summed
   uint32_t ba_n = 0;
   HardRowInfo ri_n{};
   auto fetch = [&](uint64_t r) {
     ba_n = 0;
-    if (r < q.hard_nseg) {
+    if (r < r_end) {
       ba_n = bd.hard_bact[q.hard_row0 + r];
       if (ba_n > s0) ri_n = bd.rowinfo[q.hard_row0 + r];
     }
   };
-  fetch(t);
-  for (uint64_t r0 = 0; r0 < q.hard_nseg; r0 += NT) {
+  fetch(bd.pass_r0 + t);
+  for (uint64_t r0 = bd.pass_r0; r0 < r_end; r0 += NT) {
     const uint32_t ba = ba_n;
     const HardRowInfo ri = ri_n;
     fetch(r0 + NT + t);
@@ -107,8 +113,8 @@ __device__ __forceinline__ void hard_fin_desc(const BatchDev &bd, const BlockDes
     bool act = false;
     if (ba > s0) {
       act = true;
-      const uint32_t *T = bd.hard_T + q.tm_off + ri.toff + s0;
-      const int32_t *m = bd.hard_M + q.m_off + ri.moff + s0;
+      const uint32_t *T = bd.hard_T + (q.tm_off + ri.toff - bd.pass_toff) + s0;
+      const int32_t *m = bd.hard_M + (q.m_off + ri.moff - bd.pass_moff) + s0;
       const uint32_t nj = min(ns, ba - (uint32_t)s0);   // this row's stages of the block
 #pragma unroll
       for (uint32_t j = 0; j < FS; j++)
@@ -150,7 +156,15 @@ __device__ __forceinline__ void hard_fin_desc(const BatchDev &bd, const BlockDes
     if (fs.stop) break;
   }
   i128 tt = block_sum_i128(carry, fs.red);
-  if (t == 0) st_out(&bd.parts[q.part_off + d.part], tt);
+  if (t == 0) {
+    i128 *out = &bd.parts[q.part_off + d.part];
+    st_out(out, bd.pass_add ? *out + tt : tt);
+    if (bd.hard_base) {
+#pragma unroll
+      for (uint32_t j = 0; j < FS; j++)
+        if (j < ns) bd.hard_base[s0 + j] = base[j];
+    }
+  }
 }
 
 // ------------------------------------------------------------- combine

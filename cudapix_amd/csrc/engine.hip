@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <atomic>
@@ -224,12 +226,13 @@ static int build_bootstrap(cpx_ctx *c)
 {
   if (c->small.p) return CPX_OK;
   int rc;
-  if ((rc = dbuf_reserve(c, c->small, 4096, &c->table_bytes))) return rc;
-  if ((rc = dbuf_reserve(c, c->small_primes, 24000, &c->table_bytes))) return rc;
-  HIPCHK(launch_small_sieve(c->small.p, 4096, c->stream));
-  HIPCHK(launch_extract_primes(c->small.p, (1u << 18) - 1, c->small_primes.p, 24000, c->stream));
+  // primes below 2^19 (43,390 of them): they sieve a pi table to 2^38
+  if ((rc = dbuf_reserve(c, c->small, 8192, &c->table_bytes))) return rc;
+  if ((rc = dbuf_reserve(c, c->small_primes, 44000, &c->table_bytes))) return rc;
+  HIPCHK(launch_small_sieve(c->small.p, 8192, c->stream));
+  HIPCHK(launch_extract_primes(c->small.p, (1u << 19) - 1, c->small_primes.p, 44000, c->stream));
   PiEntry last;
-  HIPCHK(hipMemcpyAsync(&last, c->small.p + 4095, sizeof(last), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&last, c->small.p + 8191, sizeof(last), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->n_small_primes = 1 + last.count + (uint32_t)__builtin_popcount(last.bits);
   return CPX_OK;
@@ -287,6 +290,12 @@ static void table_limits(uint64_t R, uint64_t P, uint64_t Y, uint64_t *Rm_, uint
   *Ym_ = Y;
 }
 
+// Largest pi-table reach: entries hold 32-bit prefix counts, and
+// pi(1.04e11) is about 4.28e9 < 2^32 (the 8192-entry rounding adds at most
+// 2^19 integers); the bootstrap primes below 2^19 sieve to 2^38.  So x <=
+// (1.04e11)^2 ~ 1.08e22 for the tables that reach sqrt x.
+static const uint64_t PITAB_MAX = 104000000000ull;
+
 bool tables_cover(const cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
 {
   uint64_t Rm, Pm, Ym;
@@ -304,13 +313,14 @@ int ensure_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
   table_limits(R, P, Y, &Rm, &Pm, &Y);
   if (!c->opts.fresh_tables && c->pitab_limit >= Rm && c->prime_limit >= Pm && c->mlpf_limit >= Y)
     return CPX_OK;
+  if (Rm > PITAB_MAX)
+    return ctx_fail(c, CPX_ERANGE, "pi table above 1.04e11 (its 32-bit prefix counts) is not supported");
   if (!c->opts.fresh_tables) {                      // headroom for later, larger queries
     Rm = std::max<uint64_t>(Rm, c->pitab_limit + c->pitab_limit / 4);
     Pm = std::max<uint64_t>(Pm, c->prime_limit + c->prime_limit / 4);
     Y = std::max<uint64_t>(Y, c->mlpf_limit + c->mlpf_limit / 4);
-    Rm = std::max<uint64_t>({Rm, Pm, Y});
+    Rm = std::min<uint64_t>(std::max<uint64_t>({Rm, Pm, Y}), PITAB_MAX);
   }
-  if ((u128)Rm >= ((u128)1 << 36)) return ctx_fail(c, CPX_ERANGE, "pi table above 2^36 is not supported");
   if (Pm >= (1ull << 32)) return ctx_fail(c, CPX_ERANGE, "prime table above 2^32 is not supported");
   free_tables(c);
   // pi table over [0, 64 * nent), nent a multiple of the 8192-entry segment
@@ -887,6 +897,7 @@ static Layout make_layout(const Caps &k)
   L.hard_M = o;    o = al(o + k.mm * 4);
   L.hard_bact = o; o = al(o + k.rows * 4);
   L.hard_acc = o;  o = al(o + k.mrows * 8);
+  L.hard_base = o; o = al(o + k.base * 8);
   L.p2_cnt = o;    o = al(o + k.p2rows * 4);
   L.p2_np = o;     o = al(o + k.p2rows * 4);
   L.p2_loc = o;    o = al(o + k.p2rows * 8);
@@ -899,7 +910,7 @@ static Layout make_layout(const Caps &k)
 
 static Caps needed_caps(const Plan &pl)
 {
-  Caps k;
+  Caps k{};
   k.q = pl.q.size(); k.stages = pl.stage_total; k.tiles = pl.tiles.size(); k.groups = pl.groups.size();
   k.ord = pl.nord; k.triv = pl.ntriv; k.easy = pl.neasy; k.fin = pl.nfin;
   k.p2d = pl.np2d;
@@ -908,20 +919,33 @@ static Caps needed_caps(const Plan &pl)
   return k;
 }
 
+// Arena capacities for `need` from the current ones `have`: every field of
+// Caps at least its need (and 64 queries, 16 groups), grown by half when it
+// grows.  Returns whether any field grew.
+static bool grown_caps(const Caps &have, const Caps &need, Caps &k)
+{
+  static_assert(sizeof(Caps) % sizeof(uint64_t) == 0, "Caps is read as an array of counts");
+  k = have;
+  Caps want = need;
+  want.q = std::max<uint64_t>(want.q, 64);
+  want.groups = std::max<uint64_t>(want.groups, 16);
+  uint64_t *kk = &k.q;
+  const uint64_t *ww = &want.q;
+  bool grow = false;
+  for (size_t f = 0; f < sizeof(Caps) / sizeof(uint64_t); f++)
+    if (ww[f] > kk[f]) {
+      kk[f] = std::max<uint64_t>(ww[f], kk[f] + kk[f] / 2);
+      grow = true;
+    }
+  return grow;
+}
+
 static int ensure_arena_caps(cpx_ctx *c, BatchSlot &S, const Caps &need, uint64_t budget)
 {
   const Caps before = S.caps;
   Caps &k = S.caps;
-  bool grow = !S.d_plan || !S.d_out;
-  auto fit = [&](uint64_t &cap, uint64_t n) {
-    if (n > cap) { cap = std::max<uint64_t>(n, cap + cap / 2); grow = true; }
-  };
-  fit(k.q, std::max<uint64_t>(need.q, 64)); fit(k.stages, need.stages); fit(k.tiles, need.tiles);
-  fit(k.groups, std::max<uint64_t>(need.groups, 16));
-  fit(k.ord, need.ord); fit(k.triv, need.triv); fit(k.easy, need.easy); fit(k.fin, need.fin);
-  fit(k.p2d, need.p2d);
-  fit(k.tm, need.tm); fit(k.rows, need.rows); fit(k.p2rows, need.p2rows); fit(k.parts, need.parts);
-  fit(k.mm, need.mm); fit(k.mrows, need.mrows);
+  bool grow = grown_caps(before, need, k);
+  if (!S.d_plan || !S.d_out) grow = true;
   if (!grow) return CPX_OK;
   Layout L = make_layout(k);
   if (budget && L.plan_bytes + L.out_bytes > budget) {
@@ -954,14 +978,14 @@ static int ensure_arena_caps(cpx_ctx *c, BatchSlot &S, const Caps &need, uint64_
   return CPX_OK;
 }
 
-static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Plan &pl)
+static int ensure_arena(cpx_ctx *c, BatchSlot &S, const Caps &need)
 {
   const uint64_t budget = c->opts.hbm_budget;
   uint64_t other = c->table_bytes + slot_bytes(c);
   if (S.d_plan && S.d_out) other -= S.L.plan_bytes + S.L.out_bytes;
   if (budget && other >= budget)
     return ctx_fail(c, CPX_ENOMEM, "batch state exceeds the context HBM budget");
-  return ensure_arena_caps(c, S, needed_caps(pl), budget ? budget - other : 0);
+  return ensure_arena_caps(c, S, need, budget ? budget - other : 0);
 }
 
 uint64_t slot_bytes(const cpx_ctx *c)
@@ -1014,6 +1038,8 @@ static BatchDev batchdev_at(const cpx_ctx *c, const Layout &L, char *P, char *O)
   b.hard_M = (int32_t *)(O + L.hard_M);
   b.hard_bact = (uint32_t *)(O + L.hard_bact);
   b.hard_acc = (int64_t *)(O + L.hard_acc);
+  b.hard_base = nullptr;                 // (a run in passes points it at L.hard_base)
+  b.pass_r0 = 0; b.pass_r1 = ~0ull; b.pass_toff = 0; b.pass_moff = 0; b.pass_add = 0; b.pass_pad = 0;
   b.p2_cnt = (uint32_t *)(O + L.p2_cnt);
   b.p2_np = (uint32_t *)(O + L.p2_np);
   b.p2_loc = (uint64_t *)(O + L.p2_loc);
@@ -1247,6 +1273,117 @@ static void choose_p2tab(const cpx_ctx *c, std::vector<HostQuery> &hq, uint32_t 
               (batch || (!c->opts.fresh_tables && c->pitab_limit >= h.zt && c->prime_limit >= h.sqrtx));
 }
 
+// ------------------------------------------------ hard segments in passes
+// A solve of one query whose hard rows (T per active stage of every
+// segment, sum(mu) per replica and active stage) exceed the pass size runs
+// its hard segments in passes: ranges of consecutive segments whose rows
+// fit (a segment's rows are never split).  Every pass's tiles fill the
+// persistent segment kernel on their own (the easy-leaf and P2-lookup
+// tiles join the first), its rows land at the start of the T and M arrays,
+// and its carry blocks continue each stage's survivor prefix from the
+// earlier passes (BatchDev::hard_base).  The other outputs (replica sums,
+// P2 rows, partial sums) are held whole.  The same multi-range carry is
+// what src/sieve/S2_hard_host.cu:115-191 does per block of segments.
+static const uint64_t PASS_BYTES_DEFAULT = 48ull << 30;
+
+struct PassPlan {
+  std::vector<uint64_t> s;                 // segment boundaries: pass k is [s[k], s[k+1])
+  std::vector<uint64_t> toff, moff;        // its first T and M element
+  std::vector<uint32_t> tstart, tcnt;      // its tiles
+  uint64_t tm_max = 0, mm_max = 0;
+  size_t n() const { return s.size() - 1; }
+};
+
+static bool plan_passes(const cpx_ctx *c, Plan &pl, PassPlan &pp)
+{
+  const uint64_t limit = c->opts.pass_bytes ? c->opts.pass_bytes : PASS_BYTES_DEFAULT;
+  if (pl.qps.size() != 1 || !pl.qps[0].d.nstage) return false;
+  const QPlan &qp = pl.qps[0];
+  if (4 * (qp.tm + qp.mm) <= limit) return false;
+  const std::vector<HardRowInfo> &R = qp.rowinfo;
+  const uint64_t n = R.size();
+  auto tend = [&](uint64_t s) { return R[s].toff + R[s].act; };
+  auto mend = [&](uint64_t s) { return R[s].moff + (uint64_t)R[s].nrep * R[s].act; };
+  pp = PassPlan{};
+  pp.s.push_back(0);
+  for (uint64_t s = 0, a = 0; s < n; s++) {
+    if (s > a && 4 * (tend(s) - R[a].toff + mend(s) - R[a].moff) > limit) {
+      pp.s.push_back(s);
+      a = s;
+    }
+  }
+  pp.s.push_back(n);
+  const size_t np = pp.n();
+  for (size_t k = 0; k < np; k++) {
+    const uint64_t a = pp.s[k], b = pp.s[k + 1] - 1;
+    pp.toff.push_back(R[a].toff);
+    pp.moff.push_back(R[a].moff);
+    pp.tm_max = std::max(pp.tm_max, tend(b) - R[a].toff);
+    pp.mm_max = std::max(pp.mm_max, mend(b) - R[a].moff);
+  }
+  // tiles regrouped by pass, in their order within a pass (a stable counting sort)
+  std::vector<uint32_t> pid(pl.tiles.size());
+  pp.tcnt.assign(np, 0);
+  for (size_t i = 0; i < pl.tiles.size(); i++) {
+    const TileDev &t = pl.tiles[i];
+    uint32_t k = 0;
+    if (t.group < CPX_TILE_P2)
+      k = (uint32_t)(std::upper_bound(pp.s.begin(), pp.s.end(), t.seg) - pp.s.begin()) - 1;
+    pid[i] = k;
+    pp.tcnt[k]++;
+  }
+  pp.tstart.assign(np, 0);
+  for (size_t k = 1; k < np; k++) pp.tstart[k] = pp.tstart[k - 1] + pp.tcnt[k - 1];
+  std::vector<uint32_t> at(pp.tstart);
+  std::vector<TileDev> tiles(pl.tiles.size());
+  for (size_t i = 0; i < pl.tiles.size(); i++) tiles[at[pid[i]]++] = pl.tiles[i];
+  pl.tiles.swap(tiles);
+  return true;
+}
+
+// The launches of a run in passes (no graphs): setup, ordinary and trivial
+// once; per pass its tile count (staged in the slot's pinned plan image,
+// so the host waits for the previous pass before it stages the next), the
+// segment kernel over its tiles and the carry blocks over its rows; then the
+// combine.
+static int launch_passes(cpx_ctx *c, BatchSlot &S, const Plan &pl, const PassPlan &pp, const BatchDev &b0,
+                         uint32_t nres_words)
+{
+  HIPCHK(hipEventRecord(S.ev[0], c->stream));
+  int rc = enqueue_part(c, b0, 0, nres_words);
+  if (rc) return rc;
+  BatchDev b = b0;
+  b.hard_base = (uint64_t *)((char *)S.d_out + S.L.hard_base);
+  HIPCHK(launch_zero_u32((uint32_t *)b.hard_base, 2 * (size_t)pl.q[0].nstage, c->stream));
+  uint32_t *cnt = (uint32_t *)((char *)S.h_plan + S.L.counters);
+  HIPCHK(hipStreamSynchronize(c->up_stream));       // the plan upload has read the image
+  HIPCHK(hipEventRecord(S.ev[2], c->stream));
+  const bool progress = getenv("CPX_PROGRESS") != nullptr;
+  const double t0 = now_ms();
+  for (size_t k = 0; k < pp.n(); k++) {
+    if (k) {
+      HIPCHK(hipStreamSynchronize(c->stream));      // the previous pass's counter copy is done
+      if (progress)
+        fprintf(stderr, "cpx: pass %zu of %zu done (segments to %llu of %llu) at %.1f s\n", k, pp.n(),
+                (unsigned long long)pp.s[k], (unsigned long long)pp.s.back(), (now_ms() - t0) / 1e3);
+    }
+    cnt[1] = pp.tcnt[k];
+    cnt[2] = 0;                                      // queue head
+    HIPCHK(hipMemcpyAsync((char *)S.d_plan + S.L.counters, cnt, 64, hipMemcpyHostToDevice, c->stream));
+    b.tiles = b0.tiles + pp.tstart[k];
+    b.pass_r0 = pp.s[k];
+    b.pass_r1 = pp.s[k + 1];
+    b.pass_toff = pp.toff[k];
+    b.pass_moff = pp.moff[k];
+    b.pass_add = k ? 1u : 0u;
+    HIPCHK(launch_segments(b, c->shape, (uint32_t)seg_grid(c), c->stream));
+    HIPCHK(launch_hard_finalize(b, GRID_FIN, c->stream));
+  }
+  HIPCHK(hipEventRecord(S.ev[3], c->stream));
+  HIPCHK(launch_combine(b0, GRID_Q, c->stream));
+  return CPX_OK;
+}
+
 static int submit_impl(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t mask, uint32_t p2mode,
                        bool use_graph, uint32_t *slot_out)
 {
@@ -1275,20 +1412,32 @@ static int submit_impl(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t
   const double tt1 = now_ms();
   Plan pl;
   if ((rc = build_plan(c, hq, mask, pl))) return rc;
-  if ((rc = ensure_arena(c, S, pl))) return rc;
+  PassPlan pp;
+  const bool passes = plan_passes(c, pl, pp);
+  Caps need = needed_caps(pl);
+  if (passes) {
+    need.tm = pp.tm_max;
+    need.mm = pp.mm_max;
+    need.base = pl.q[0].nstage;
+  }
+  if ((rc = ensure_arena(c, S, need))) return rc;
   const double tp = now_ms();
   if ((rc = upload_plan(c, S, pl))) return rc;
   BatchDev b = make_batchdev(c, S);
-  const bool graph = use_graph && c->opts.use_graphs;
+  const bool graph = use_graph && c->opts.use_graphs && !passes;
   // results region of the arena, in 32-bit words (capacity, so graphs stay valid)
   const uint32_t nres_words = (uint32_t)(S.caps.q * sizeof(ResultDev) / 4);
   if (graph && (rc = ensure_graph(c, S, b, nres_words))) return rc;
-  HIPCHK(hipEventRecord(S.ev[0], c->stream));
-  for (int part = 0; part < 3; part++) {
-    if (part == 1) HIPCHK(hipEventRecord(S.ev[2], c->stream));
-    if (graph) HIPCHK(hipGraphLaunch(S.gexec[part], c->stream));
-    else if ((rc = enqueue_part(c, b, part, nres_words))) return rc;
-    if (part == 1) HIPCHK(hipEventRecord(S.ev[3], c->stream));
+  if (passes) {
+    if ((rc = launch_passes(c, S, pl, pp, b, nres_words))) return rc;
+  } else {
+    HIPCHK(hipEventRecord(S.ev[0], c->stream));
+    for (int part = 0; part < 3; part++) {
+      if (part == 1) HIPCHK(hipEventRecord(S.ev[2], c->stream));
+      if (graph) HIPCHK(hipGraphLaunch(S.gexec[part], c->stream));
+      else if ((rc = enqueue_part(c, b, part, nres_words))) return rc;
+      if (part == 1) HIPCHK(hipEventRecord(S.ev[3], c->stream));
+    }
   }
   HIPCHK(hipMemcpyAsync(S.h_res, (char *)S.d_out + S.L.results, hq.size() * sizeof(ResultDev),
                         hipMemcpyDeviceToHost, c->stream));
@@ -1300,6 +1449,7 @@ static int submit_impl(cpx_ctx *c, const std::vector<HostQuery> &hq_in, uint32_t
   S.ms_tables = tt1 - tt0;
   S.ms_plan = (tp - t0) - (tt1 - tt0);
   S.tiles = pl.tiles.size();
+  S.passes = passes ? (uint32_t)pp.n() : 1u;
   S.hard_segments = pl.hard_segments;
   S.p2_segments = pl.p2_segments;
   *slot_out = si;
@@ -1344,6 +1494,7 @@ int collect_queries(cpx_ctx *c, uint32_t si, std::vector<ResultDev> &res, cpx_ba
     st->p2_segments = S.p2_segments;
     st->seg_threads = (uint32_t)c->shape.threads;
     st->seg_kwords = (uint32_t)c->shape.kwords;
+    st->passes = S.passes;
   }
   return CPX_OK;
 }

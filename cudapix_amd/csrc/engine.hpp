@@ -39,11 +39,12 @@ enum : uint32_t {
 // the pointers baked into a captured graph stay valid until a capacity grows.
 struct Caps {
   uint64_t q, stages, tiles, groups, ord, triv, easy, fin, p2d, tm, rows, p2rows, parts, mm, mrows;
+  uint64_t base;   // per-stage survivor totals of a run in passes (0 otherwise)
 };
 
 struct Layout {
   size_t counters, queries, stages, tiles, groups, ord, triv, easy, fin, p2d, rowinfo, plan_bytes;
-  size_t hard_T, hard_M, hard_bact, hard_acc, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
+  size_t hard_T, hard_M, hard_bact, hard_acc, hard_base, p2_cnt, p2_np, p2_loc, parts, results, out_bytes;
 };
 
 // Batch slots per context: one batch can be planned and uploaded while the
@@ -71,6 +72,7 @@ struct BatchSlot {
   size_t nq = 0;
   double t_submit = 0, ms_tables = 0, ms_plan = 0;
   uint64_t tiles = 0, hard_segments = 0, p2_segments = 0;
+  uint32_t passes = 1;
 };
 
 struct cpx_ctx {
@@ -83,7 +85,7 @@ struct cpx_ctx {
   std::recursive_mutex mu;
 
   // ---- resident tables
-  DBuf<PiEntry> small;              // pi table of [0, 2^18)
+  DBuf<PiEntry> small;              // pi table of [0, 2^19)
   DBuf<uint32_t> small_primes;
   uint32_t n_small_primes = 0;
   DBuf<PiEntry> pitab;

@@ -876,8 +876,8 @@ __device__ __forceinline__ void hard_segment(const BatchDev &bd, const TileDev &
         const HardRowInfo ri = bd.rowinfo[row];
         m.y = q.y;
         m.stage_off = q.stage_off;
-        m.toff = q.tm_off + ri.toff;
-        m.moff = q.m_off + ri.moff + (uint64_t)rep * ri.act;
+        m.toff = q.tm_off + ri.toff - bd.pass_toff;
+        m.moff = q.m_off + ri.moff - bd.pass_moff + (uint64_t)rep * ri.act;
         m.acc_slot = q.mrow_base + ri.mrow + rep;
         m.row = row;
         m.nst = min(q.nstage, ri.act);

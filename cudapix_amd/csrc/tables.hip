@@ -11,10 +11,10 @@
 #include "cpx_device.hpp"
 
 // ------------------------------------------------------------ small sieve
-// One workgroup: pi-table entries for [0, 64 * nent), nent <= 4096.
+// One workgroup: pi-table entries for [0, 64 * nent), nent <= 8192.
 __global__ __launch_bounds__(256) void k_small_sieve(PiEntry *out, uint32_t nent)
 {
-  __shared__ uint32_t bm[4096];
+  __shared__ uint32_t bm[8192];
   __shared__ uint32_t wsum[4];
   const uint32_t t = threadIdx.x, T = blockDim.x;
   for (uint32_t w = t; w < nent; w += T) bm[w] = 0xffffffffu;
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_small_sieve(PiEntry *out, uint32_t nent
 
 hipError_t launch_small_sieve(PiEntry *out, uint64_t nent, hipStream_t s)
 {
-  if (nent > 4096) return hipErrorInvalidValue;
+  if (nent > 8192) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_small_sieve, dim3(1), dim3(256), 0, s, out, (uint32_t)nent);
   return hipGetLastError();
 }

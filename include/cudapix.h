@@ -60,6 +60,10 @@ typedef struct {
   uint64_t hbm_budget;     /* bytes the context may hold on the device (0: 90% of free) */
   uint32_t fresh_tables;   /* 1: rebuild every table for each solve (one-at-a-time plain mode) */
   uint32_t use_graphs;     /* 1 (default): replay batch launches from a captured hipGraph */
+  uint64_t pass_bytes;     /* a single solve whose hard-leaf rows (4 bytes per active stage per
+                              segment, and per replica) exceed this runs its hard segments in
+                              passes over segment ranges, each range's rows within it
+                              (0: 48 GiB); results are the same */
 } cpx_opts;
 
 /* Per-solve breakdown (all exact). */
@@ -242,6 +246,9 @@ typedef struct {
   double ms_segments;      /* device time of the persistent segment kernel */
   uint64_t tiles, hard_segments, p2_segments;
   uint32_t seg_threads, seg_kwords;
+  uint32_t passes;         /* passes over segment ranges (1 unless a single query's hard rows
+                              exceed cpx_opts.pass_bytes) */
+  uint32_t reserved;
 } cpx_batch_stats;
 
 /* Many queries in one launch sequence over shared resident tables: every

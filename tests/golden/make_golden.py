@@ -173,7 +173,7 @@ def main():
             0, 4, 25, 168, 1229, 9592, 78498, 664579, 5761455, 50847534, 455052511,
             4118054813, 37607912018, 346065536839, 3204941750802, 29844570422669,
             279238341033925, 2623557157654233, 24739954287740860, 234057667276344607,
-            2220819602560918840, 21127269486018731928])},
+            2220819602560918840, 21127269486018731928, 201467286689315906290])},
         "pi2": {"10": 172, "20": 82025, "30": 54400028, "32": 203280221, "40": 41203088796,
                 "64": 425656284035217743},
         "mertens10": {"1": -1, "2": 1, "3": 2, "4": -23, "5": -48, "6": 212, "7": 1037,

@@ -9,9 +9,12 @@
 // or packing fails the run; on top of that every index the kernels will form
 // from the plan is checked against the arena capacities.
 //
-// stdin: one batch per line, "mask p2mode x[:y] x[:y] ..." (decimal x < 2^100).
+// stdin: one batch per line, "[pass=<bytes>] mask p2mode x[:y] x[:y] ..."
+// (decimal x < 2^100; pass= sets cpx_opts.pass_bytes, so a single query
+// whose hard rows exceed it is planned in passes over segment ranges, whose
+// windows are checked as the kernels will index them).
 // stdout: one line per batch, "ok <queries> <tiles> <plan_bytes> <out_bytes>
-// <estimate_bytes_sum>", or "FAIL <what>" (and exit status 1).
+// <estimate_bytes_sum> <passes>", or "FAIL <what>" (and exit status 1).
 #define CPX_NO_ROCTX 1
 #include "../cudapix_amd/csrc/engine.hip"
 #include <cstdio>
@@ -79,7 +82,7 @@ static void fake_tables(cpx_ctx *c, uint64_t R, uint64_t P, uint64_t Y)
   c->mlpf_limit = std::max<uint64_t>(c->mlpf_limit, (Y + 1) / 2 * 2 + 1);
 }
 
-static void check_plan(const cpx_ctx *c, const Plan &pl, const Layout &L, const Caps &k)
+static void check_plan(const cpx_ctx *c, const Plan &pl, const Layout &L, const Caps &k, bool passes)
 {
   const uint64_t nq = pl.q.size();
   for (size_t i = 0; i < nq; i++) {
@@ -90,8 +93,10 @@ static void check_plan(const cpx_ctx *c, const Plan &pl, const Layout &L, const 
     CHECK(d.hard_row0 + d.hard_nseg <= k.rows, "rows");
     CHECK(d.p2_row0 + d.p2_nseg <= k.p2rows, "p2 rows");
     CHECK(d.mrow_base + d.hard_nmrow <= k.mrows, "m rows");
-    CHECK(d.tm_off + qp.tm <= k.tm, "T entries");
-    CHECK(d.m_off + qp.mm <= k.mm, "M entries");
+    if (!passes) {                    // (in passes the T and M arrays hold one range's rows)
+      CHECK(d.tm_off + qp.tm <= k.tm, "T entries");
+      CHECK(d.m_off + qp.mm <= k.mm, "M entries");
+    }
     CHECK(qp.rowinfo.size() == d.hard_nseg, "row count");
     CHECK(d.n_ord_parts + d.n_triv_parts + d.n_easy_parts + d.n_hard_parts + d.n_p2_parts == qp.parts,
           "part split");
@@ -136,6 +141,41 @@ static void check_plan(const cpx_ctx *c, const Plan &pl, const Layout &L, const 
   CHECK(L.plan_bytes > 0, "layout");
 }
 
+// A run in passes: the ranges cover the segments in order; every tile sits
+// in the pass of its segment (term tiles in the first), once; and every row
+// a pass's tiles and carry blocks touch, T at tm_off + toff - pass_toff + b
+// and M at m_off + moff - pass_moff + r * act + b (b < act, r < nrep), lies
+// inside the arrays sized for the largest range.
+static void check_passes(const Plan &pl, const PassPlan &pp, const Caps &k)
+{
+  CHECK(pl.qps.size() == 1, "passes: one query");
+  const QueryDev &d = pl.q[0];
+  const QPlan &qp = pl.qps[0];
+  const size_t np = pp.n();
+  CHECK(np >= 2 && pp.s.front() == 0 && pp.s.back() == d.hard_nseg, "passes: cover the segments");
+  CHECK(pp.toff.size() == np && pp.moff.size() == np && pp.tstart.size() == np && pp.tcnt.size() == np,
+        "passes: per-pass records");
+  uint64_t ntile = 0;
+  for (size_t p = 0; p < np; p++) {
+    CHECK(pp.s[p] < pp.s[p + 1], "passes: ranges ascend");
+    CHECK(pp.tstart[p] == ntile, "passes: tiles contiguous");
+    ntile += pp.tcnt[p];
+    for (uint64_t s = pp.s[p]; s < pp.s[p + 1]; s++) {
+      const HardRowInfo &ri = qp.rowinfo[s];
+      CHECK(ri.toff >= pp.toff[p] && d.tm_off + ri.toff - pp.toff[p] + ri.act <= k.tm, "passes: T window");
+      CHECK(ri.moff >= pp.moff[p] && d.m_off + ri.moff - pp.moff[p] + (uint64_t)ri.nrep * ri.act <= k.mm,
+            "passes: M window");
+    }
+    for (uint64_t i = pp.tstart[p]; i < pp.tstart[p] + pp.tcnt[p]; i++) {
+      const TileDev &t = pl.tiles[i];
+      if (t.group == CPX_TILE_EASY || t.group == CPX_TILE_P2) CHECK(p == 0, "passes: term tiles first");
+      else CHECK(t.seg >= pp.s[p] && t.seg < pp.s[p + 1], "passes: tile in its range");
+    }
+  }
+  CHECK(ntile == pl.tiles.size(), "passes: every tile once");
+  CHECK(k.base >= d.nstage, "passes: stage totals");
+}
+
 int main()
 {
   cpx_ctx ctx;
@@ -149,6 +189,12 @@ int main()
     if (line.empty()) continue;
     std::istringstream in(line);
     uint32_t mask, p2mode;
+    c->opts.pass_bytes = 0;
+    if (line.compare(0, 5, "pass=") == 0) {
+      std::string pb;
+      in >> pb;
+      c->opts.pass_bytes = (uint64_t)parse_u128(pb.substr(5));
+    }
     in >> mask >> p2mode;
     std::vector<HostQuery> hq;
     std::string tok;
@@ -167,15 +213,26 @@ int main()
     fake_tables(c, R, P, Y);
     Plan pl;
     CHECK(build_plan(c, hq, mask, pl) == CPX_OK, "build_plan");
-    const Caps k = needed_caps(pl);
+    PassPlan pp;
+    const bool passes = plan_passes(c, pl, pp);
+    Caps need = needed_caps(pl);
+    if (passes) {                     // as submit_impl asks for the arena
+      need.tm = pp.tm_max;
+      need.mm = pp.mm_max;
+      need.base = pl.q[0].nstage;
+    }
+    Caps k{};                         // and as a fresh slot's arena grows to it
+    grown_caps(Caps{}, need, k);
     const Layout L = make_layout(k);
     std::vector<char> image(L.plan_bytes);
     pack_plan(L, pl, image.data());
-    check_plan(c, pl, L, k);
+    check_plan(c, pl, L, k, passes);
+    if (passes) check_passes(pl, pp, k);
     uint64_t est = 0;
     for (const HostQuery &h : hq) est += plan_bytes_estimate(c, h);
-    std::printf("ok %zu %zu %zu %zu %llu\n", hq.size(), pl.tiles.size(), L.plan_bytes, L.out_bytes,
-                (unsigned long long)est);
+    const Layout Ln = make_layout(need);   // the bytes the plan itself needs
+    std::printf("ok %zu %zu %zu %zu %llu %zu\n", hq.size(), pl.tiles.size(), Ln.plan_bytes, Ln.out_bytes,
+                (unsigned long long)est, passes ? pp.n() : (size_t)1);
   }
   return 0;
 }

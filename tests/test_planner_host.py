@@ -42,7 +42,7 @@ def run(exe, lines):
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-4000:]
     out = [ln.split() for ln in p.stdout.splitlines()]
     assert len(out) == len(lines) and all(o[0] == "ok" for o in out), p.stdout
-    return [dict(zip(("queries", "tiles", "plan", "out", "estimate"), map(int, o[1:]))) for o in out]
+    return [dict(zip(("queries", "tiles", "plan", "out", "estimate", "passes"), map(int, o[1:]))) for o in out]
 
 
 def test_single_term_plans(checker, oracle):
@@ -72,3 +72,18 @@ def test_single_query_plans_and_estimate(checker):
     for x, v in zip(xs, r):
         if x <= 10 ** 17:
             assert v["estimate"] >= v["plan"] + v["out"], (x, v)
+
+
+def test_pass_plans(checker):
+    # a single query's hard segments in passes over segment ranges: forced
+    # small pass sizes (down to one segment per pass) and 1e20 at 1 GB; the
+    # checker verifies that the ranges cover the segments, every tile sits in
+    # its range's list once, and every T and M row a pass touches lies in the
+    # arrays sized for the largest range
+    cases = [(10 ** 10, 4), (10 ** 12, 2048), (10 ** 14, 65536), (10 ** 16, 1 << 20),
+             (2 ** 64 + 1, 1 << 28), (10 ** 20, 10 ** 9)]
+    r = run(checker, ["pass=%d %d 0 %d" % (b, ALL, x) for x, b in cases])
+    assert all(v["passes"] > 1 for v in r), r
+    # the default pass size leaves these in one pass
+    r = run(checker, ["%d 0 %d" % (ALL, x) for x, _ in cases[:4]])
+    assert all(v["passes"] == 1 for v in r), r
