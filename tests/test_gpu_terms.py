@@ -73,3 +73,25 @@ def test_s2_sigma_form_matches_oracle(ctx, oracle, x, alpha):
         assert g[k] == r[k], k
     v = ctx.V(x, y)
     assert (g["S2"] == v) == g["exact"], (g["S2"], v, g["exact"])
+
+
+def _iroot(n, k):
+    r = int(round(n ** (1.0 / k)))
+    while r ** k > n:
+        r -= 1
+    while (r + 1) ** k <= n:
+        r += 1
+    return r
+
+
+@pytest.mark.parametrize("x", [10 ** 18, 2 ** 64 + 1, 10 ** 20])
+def test_s2_sigma_form_equals_v_beyond_the_oracle(ctx, x):
+    """Past the oracle's reach: with y > x^(3/8), x/y^2 < x^(1/4), so no prime
+    lies in (x^1/4, x/y^2] and the sigma form's sum must equal the direct V of
+    the same y exactly -- two partitions of the easy leaves, each on the GPU
+    (the 128-bit path at 2^64 + 1)."""
+    y = _iroot(x ** 3, 8) + 1
+    print("case sigma x=%d y=%d" % (x, y), flush=True)
+    g = ctx.s2_sigma(x, y)
+    assert g["y"] == y and g["exact"]
+    assert g["S2"] == ctx.V(x, y)
