@@ -63,7 +63,9 @@ typedef struct {
   uint64_t pass_bytes;     /* a single solve whose hard-leaf rows (4 bytes per active stage per
                               segment, and per replica) exceed this runs its hard segments in
                               passes over segment ranges, each range's rows within it
-                              (0: 48 GiB); results are the same */
+                              (0: 48 GiB); results are the same.  Such a solve waits for
+                              each pass before it queues the next, so cpx_pi_batch_submit
+                              returns only when its last pass is queued */
 } cpx_opts;
 
 /* Per-solve breakdown (all exact). */
