@@ -2,8 +2,9 @@
  * oracle.c -- CPU restatement of the Deleglise-Rivat combinatorial pi(x) terms.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links or calls this
- * file.  It is used by tests/ (as the checker), by __graft_entry__.smoke() (as
- * the checker) and by bench.py's cpu_baseline leg (timed beside the GPU).
+ * file.  It is used by tests/ and by __graft_entry__.smoke() (as the values the
+ * GPU results are compared with) and by bench.py's cpu_baseline leg (timed
+ * beside the GPU).
  *
  * Every term is restated from the reference's algorithm (file:line below,
  * relative to the reference tree) with its known defects fixed, and with every

@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
+bench.py's cpu_baseline leg, always as the values GPU results are compared with or
+the timed CPU baseline,
 never by the shipped library.  See oracle.c for the reference file:line each
 term restates.
 """
