@@ -61,6 +61,8 @@ struct QueryDev {
   uint32_t n_ord_parts, n_triv_parts, n_easy_parts, n_hard_parts;
   uint32_t n_p2_parts;                // P2 by table lookups: partial sums of pi(x/p) (after the hard ones)
   uint32_t p2tab;                     // P2 from table lookups instead of P2 segment rows
+  uint32_t omega;                     // the hard leaves with x/(pq) < p^2 go to lookups (below)
+  uint32_t pad_o;
   uint64_t pi_e;                      // written by setup: pi(p2_e)
   uint64_t pi_sqrtx;                  // written by setup: pi(sqrt x)
 };
@@ -75,6 +77,10 @@ struct StageDev {
   uint32_t ylo;        // floor(y / p)
   uint32_t xph;        // floor(x / p) >> 64 (nonzero only for the first stages of x above 2^68)
   uint32_t pad;
+  // the largest leaf prime q the segment sieve counts at a sparse stage (p^2 > y)
+  // of a query with `omega`: min(y, floor(x / p^3)); the leaves with q above it
+  // have x/(pq) < p^2 and go to pi-table lookups; ~0 otherwise
+  uint64_t xp3;
 };
 
 // floor(a / d) from a precomputed inv ~ 1/d: exact when a < 2^53 or the

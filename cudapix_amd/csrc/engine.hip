@@ -455,6 +455,8 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   QueryDev &d = qp.d;
   d.x_lo = (uint64_t)h.x;
   d.x_hi = (uint64_t)(h.x >> 64);
+  d.omega = 0;
+  d.pad_o = 0;
   if (h.small) {
     d.small = 1;
     return;
@@ -520,6 +522,34 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
   // hard carry: one block per CPX_FIN_STAGES stages
   for (uint64_t b = 0; b < d.nstage; b += CPX_FIN_STAGES)
     qp.fin.push_back({i, part++, b, std::min<uint64_t>(d.nstage, b + CPX_FIN_STAGES)});
+  // hard leaves with x/(pq) < p^2: at the sparse stages (p^2 > y) every leaf
+  // is a prime q in (p, y], and for q > x/p^3 its phi(x/(pq), pi(p) - 1) is
+  // pi(x/(pq)) - pi(p) + 2, a pi-table lookup as V's leaves are.  The segment
+  // sieve counts only q <= min(y, x/p^3) (StageDev::xp3); the rest go to
+  // easy-leaf tiles whose descriptors sit in the hard part group (the tile
+  // negates their sums: the hard term is minus the sum of mu * phi, and
+  // mu(q) = -1).  The tiles take x/p in 64 bits, so x/p < 2^64 is required
+  // at the first such p.  About half of all hard leaves at 1e13..1e16.
+  if (d.nstage) {
+    const uint64_t sy = cpx_isqrt128((u128)h.y);                         // p > sy <=> p^2 > y
+    const uint64_t js = std::max<uint64_t>(h.c, host_pi(c, sy));         // first such stage prime
+    if (js < d.B && ((h.x / c->h_primes[js]) >> 64) == 0) {
+      d.omega = 1;
+      const uint64_t ky = d.a;                                          // pi(y)
+      uint64_t j0 = js, work = 0;
+      for (uint64_t j = js; j < d.B; j++) {
+        const uint64_t p = c->h_primes[j];
+        const u128 t3 = h.x / ((u128)p * p * p);
+        const uint64_t k0 = std::max<uint64_t>(j + 1, t3 >= h.y ? ky : host_pi(c, (uint64_t)t3));
+        work += (ky > k0 ? ky - k0 : 0) + 64;
+        if (work >= EASY_WORK || j + 1 == d.B) {
+          qp.easy.push_back({i, part++, j0, j + 1});
+          j0 = j + 1;
+          work = 0;
+        }
+      }
+    }
+  }
   d.n_hard_parts = part - d.n_ord_parts - d.n_triv_parts - d.n_easy_parts;
   // P2 by lookups: prime indices (pi(y), pi(sqrt x)] (primes[k] is the (k+1)-th prime)
   if (d.p2tab) {
@@ -560,10 +590,11 @@ static void plan_query(const cpx_ctx *c, const HostQuery &h, uint32_t i, uint32_
     for (uint32_t k = 0; est && k < act; k += kstep) {
       const double p = (double)c->h_primes[h.c + k];
       const double xp = (double)xps[k];
-      const double mhi = lo ? std::min(fy, std::floor(xp * inv_lo)) : fy;
+      double mhi = lo ? std::min(fy, std::floor(xp * inv_lo)) : fy;
       double mlo = std::max(std::floor(xp * inv_hi), std::floor(fy / p));
       const bool sparse = p * p > fy;
       if (sparse) mlo = std::max(mlo, p);
+      if (sparse && d.omega) mhi = std::min(mhi, std::floor(xp / (p * p)));
       if (mhi > mlo) {
         leafy += kstep;
         leaves += kstep * (sparse ? (mhi - mlo) / std::log(mhi) : 0.5 * (mhi - mlo));
@@ -1121,6 +1152,11 @@ static void pack_stages(const cpx_ctx *c, const Layout &L, const Plan &pl, char 
       d.xp = (uint64_t)xq;
       d.xph = (uint32_t)(xq >> 64);
       d.pad = 0;
+      d.xp3 = ~0ull;
+      if (q.omega && (uint64_t)p * p > q.y) {
+        const u128 t = xq / ((uint64_t)p * p);
+        d.xp3 = t < (u128)q.y ? (uint64_t)t : q.y;
+      }
       st[q.stage_off + k] = d;
     }
   });

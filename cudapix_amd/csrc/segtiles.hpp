@@ -239,6 +239,7 @@ __device__ __forceinline__ void hard_chunk_table(const BatchDev &bd, uint32_t c,
       uint64_t mlo = ~0ull;
       if (cpx_quot_below_2_50(xp, xph, hi))
         mlo = xph ? cpx_div_recip_w(xp, xph, hi, inv_hi) : cpx_div_recip(xp, hi, inv_hi);
+      if (sparse && mhi > sd.xp3) mhi = sd.xp3;        // q above x/p^3: pi-table lookups
       if (mlo < mhi) {
         if (mlo < sd.ylo) mlo = sd.ylo;
         if (sparse && mlo < p) mlo = p;
@@ -940,6 +941,7 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
   constexpr uint32_t SBT = T < 256 ? T : 256, SUB = T / SBT, BIG = 8 * U * T;
   uint64_t *sxp = reinterpret_cast<uint64_t *>(lds);    // x / p of the chunk's primes
   uint32_t *sk1 = lds + 2 * T;                          // pi(min(y, x / p^2)): q indices end here
+  uint32_t *sk0 = lds + 3 * T;                          // max(j + 1, pi(x / p^3)): and start here
   const uint32_t t = threadIdx.x;
   const BlockDesc d = bd.easy_desc[di];
   const QueryDev &q = bd.queries[d.query];
@@ -957,9 +959,13 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
         if (v < qmax) qmax = v;
       }
       const uint64_t k1 = pi_lookup(bd.pitab, qmax);
+      // q > x/p^3 (x/(pq) < p^2): no bound above x^(1/4), where x/p^3 < p
+      const uint64_t t3 = (xp / p) / p;
+      const uint64_t k0 = t3 > p ? max(j + 1, pi_lookup(bd.pitab, t3 < q.y ? t3 : q.y)) : j + 1;
       sxp[t] = xp;
       sk1[t] = (uint32_t)k1;
-      if (k1 > j + 1) sum += (i128)(k1 - j - 1) * (1 - (i128)j);   // (2 - pi(p)) per q, pi(p) = j + 1
+      sk0[t] = (uint32_t)k0;
+      if (k1 > k0) sum += (i128)(k1 - k0) * (1 - (i128)j);     // (2 - pi(p)) per q, pi(p) = j + 1
     }
     __syncthreads();
     // primes with at least BIG q's take the whole block one at a time (U * T
@@ -969,13 +975,13 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
     for (uint32_t pass = 0; pass < 2; pass++) {
       const uint32_t nthr = pass ? SBT : T, stride = U * nthr, tl = pass ? t % SBT : t;
       for (uint32_t jj = pass ? t / SBT : 0; jj < nj; jj += pass ? SUB : 1) {
-        const uint64_t j = j0 + jj, k1 = sk1[jj], xp = sxp[jj];
-        if (k1 <= j + 1 || ((k1 - j - 1 >= BIG) != (pass == 0))) continue;
+        const uint64_t k1 = sk1[jj], k0 = sk0[jj], xp = sxp[jj];
+        if (k1 <= k0 || ((k1 - k0 >= BIG) != (pass == 0))) continue;
         // a chain past k1 repeats index k and is masked; x/(p q) < x/p^2 <
         // 2^50, so the reciprocal divide with the tabulated 1/q is exact
         const double xpf = (double)xp;
         uint64_t s = 0;
-        for (uint64_t k = j + 1 + tl; k < k1; k += stride) {
+        for (uint64_t k = k0 + tl; k < k1; k += stride) {
           uint32_t qv[U];
           double rv[U];
           bool ok[U];
@@ -1013,7 +1019,9 @@ __device__ __forceinline__ void easy_tile(const BatchDev &bd, uint32_t di, uint3
   if (t == 0) {
     i128 tot = 0;
     for (uint32_t w = 0; w < T / CPX_WAVE; w++) tot += red[w];
-    st_out(&bd.parts[q.part_off + d.part], (i128)tot);
+    // a descriptor of hard leaves (its part in the hard group) adds mu * phi
+    const bool hard = d.part >= q.n_ord_parts + q.n_triv_parts + q.n_easy_parts;
+    st_out(&bd.parts[q.part_off + d.part], hard ? -tot : tot);
   }
   (void)sh;
 }

@@ -47,6 +47,12 @@ __global__ __launch_bounds__(256) void k_setup(BatchDev bd)
       d.xp = (uint64_t)xq;
       d.xph = (uint32_t)(xq >> 64);
       d.pad = 0;
+      d.xp3 = ~0ull;
+      if (q.omega && (uint64_t)p * p > q.y) {
+        uint64_t r3;
+        const u128 t = cpx_div128_64(xq, (uint64_t)p * p, &r3);
+        d.xp3 = t < (u128)q.y ? (uint64_t)t : q.y;
+      }
       bd.stages[q.stage_off + s] = d;
     }
   }
